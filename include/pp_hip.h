@@ -1,0 +1,164 @@
+/*
+ * pp_hip.h -- C ABI of the MI355X rotation-layer engine (libpp_hip.so).
+ *
+ * The drop-in boundary for the reference's device backend of Pauli propagation
+ * (lkdvos/paulistrings-rs, crates/paulistrings/src/engine/gpu/): a sum resident
+ * on one GPU, stepped through Pauli-rotation channels under a built-in
+ * truncation policy, read back in the reference's canonical order.  Plain
+ * pointers and sizes only; a Rust, ctypes or cgo host binds it as-is
+ * (INTEGRATION.md shows the bindings).
+ *
+ * Citations are into crates/paulistrings/src/ of the reference.
+ *
+ * Conventions:
+ *   - keys are the symplectic encoding of pauli_string.rs:15-20: term i's x
+ *     words are x[i*W .. i*W+W), its z words z[i*W .. i*W+W) (term-major, as
+ *     the reference's x_array()/z_array() hand them out); W is 1 (<= 64 qubits)
+ *     or 2 (<= 128 qubits);
+ *   - coefficients are float64 or float32 (coefficient.rs:199-200);
+ *   - calls on one context are serialized by the caller; work inside a call is
+ *     asynchronous on the context's stream and pp_sync() waits for it;
+ *   - every entry point returns a pp_status; pp_last_error() explains the last
+ *     failure on a context.
+ */
+#ifndef PP_HIP_H
+#define PP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, mapping the reference's GpuError (engine/gpu/error.rs:9-75). */
+typedef enum {
+    PP_OK = 0,
+    PP_ERR_NO_DEVICE = 1,        /* GpuError::NoDevice */
+    PP_ERR_OUT_OF_MEMORY = 2,    /* GpuError::OutOfMemory { device, bytes } */
+    PP_ERR_UNSUPPORTED = 3,      /* GpuError::Unsupported(reason) */
+    PP_ERR_DRIVER = 4,           /* GpuError::Driver(hipError_t) */
+    PP_ERR_INVALID_ARGUMENT = 5
+} pp_status;
+
+typedef enum { PP_COEFF_F64 = 0, PP_COEFF_F32 = 1 } pp_coeff_kind;
+
+typedef enum { PP_FORWARD = 0, PP_HEISENBERG = 1 } pp_direction; /* engine/mod.rs Direction */
+
+/* A Pauli rotation exp(-i theta P / 2) about P = (gx, gz): PauliRotation::new
+ * (channel/rotation.rs:15-25).  Words beyond W are ignored. */
+typedef struct {
+    uint64_t gx[4];
+    uint64_t gz[4];
+    double theta;
+} pp_channel;
+
+/* A built-in truncation policy (truncation/builtin.rs, truncation/tree.rs):
+ * the per-term filter as a postfix program of at most 15 nodes (the reference's
+ * device KeepProgram, engine/gpu/truncation.rs:8-102), and an optional TopN
+ * layer pass after every layer (TopN::finalize_layer, builtin.rs:73-129). */
+typedef enum {
+    PP_KEEP_TRUE = 0,
+    PP_KEEP_COEFF = 1,  /* CoefficientThreshold(eps): eps < 0 or |c|^2 > eps^2 */
+    PP_KEEP_WEIGHT = 2, /* WeightCutoff(k): Pauli weight <= k */
+    PP_KEEP_AND = 3,
+    PP_KEEP_OR = 4
+} pp_keep_op;
+
+typedef struct {
+    int32_t op;
+    uint32_t k;
+    double eps;
+} pp_keep_node;
+
+typedef struct {
+    int32_t n_nodes; /* 0: keep every nonzero term */
+    pp_keep_node nodes[15];
+    int32_t has_topn;
+    uint64_t topn;
+} pp_policy;
+
+/* PropagateOptions (engine/mod.rs:62-106): the reference's bucket-count knobs,
+ * which decide its canonical order and TopN's tie order. */
+typedef struct {
+    uint32_t target_bucket_len; /* default 1024 */
+    uint32_t min_buckets;       /* default 128 */
+} pp_options;
+
+typedef struct pp_context pp_context;
+typedef struct pp_sum pp_sum;
+
+/* ---- context ------------------------------------------------------------------------------ */
+
+/* One device and one stream (a lone GpuSum's own context, engine/gpu/sum.rs:71-80). */
+pp_status pp_context_create(int device, pp_context** out);
+void pp_context_destroy(pp_context* ctx);
+const char* pp_last_error(const pp_context* ctx);
+pp_status pp_sync(pp_context* ctx);
+/* The hipStream_t every call of this context is queued on. */
+pp_status pp_context_stream(pp_context* ctx, void** stream);
+
+/* ---- resident sums (GpuSum, engine/gpu/sum.rs:47-253) ----------------------------------------- */
+
+/* Upload n terms with distinct keys (a PauliSum's terms), hash seeded by `hash_seed`
+ * (DEFAULT_HASH_SEED = 0x9E3779B97F4A7C15, pauli_sum/storage.rs:12).  Exact zeros are dropped.
+ * The sum starts at the reference's bucket count desired_bits(n, 1024, 128) (pauli_sum/accumulator.rs:49-52). */
+pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp_coeff_kind kind, uint64_t n,
+                        const uint64_t* x, const uint64_t* z, const void* coeff, uint64_t hash_seed,
+                        pp_sum** out);
+/* A device-side copy (same context), e.g. to reset a benchmark input without a host round trip. */
+pp_status pp_sum_clone(const pp_sum* src, pp_sum** out);
+pp_status pp_sum_free(pp_sum* sum);
+pp_status pp_sum_len(pp_sum* sum, uint64_t* len);
+/* Download in canonical order: the reference's bucket index ascending, then lexicographic (x, z)
+ * (pauli_sum/storage.rs:285-295, GpuSum::to_host engine/gpu/sum.rs:216-253).  Arrays hold len terms. */
+pp_status pp_sum_download(pp_sum* sum, uint64_t* x, uint64_t* z, void* coeff);
+/* The reference's bucket bits for this sum, and the device's own. */
+pp_status pp_sum_bucket_bits(pp_sum* sum, int32_t* reference_bits, int32_t* device_bits);
+
+/* ---- layers -------------------------------------------------------------------------------- */
+
+/* propagate (engine/mod.rs:121-187 -> engine/driver.rs:319-592): every channel in circuit order
+ * (reversed, each adjoint, for PP_HEISENBERG), each layer's outputs kept iff nonzero and the policy's
+ * per-term filter passes, the TopN pass after every layer.  layer_len (optional, n_channels entries)
+ * receives the length after every layer.  options may be NULL for the defaults. */
+pp_status pp_propagate(pp_sum* sum, const pp_channel* channels, uint64_t n_channels, pp_direction direction,
+                       const pp_policy* policy, const pp_options* options, uint64_t* layer_len);
+
+/* One rotation layer on the device's current bucketing, with no rebucket and no layer pass: the
+ * kernel the benchmark times (DevicePartition::apply_layer, engine/gpu/partition.rs:315-374). */
+pp_status pp_apply_rotation(pp_sum* sum, const pp_channel* channel, int adjoint, const pp_policy* policy);
+
+/* The same layer out of place: dst receives the layer applied to src, which is left as it was (dst's previous
+ * terms are discarded).  Asynchronous: no host synchronization; pp_sum_len(dst) waits for the counts.  For
+ * stepping many layers from one resident input, e.g. every channel of a dense layer on the same saturated sum.
+ * Both sums belong to one context and share width, coefficient type, qubit count and hash seed. */
+pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* channel, int adjoint,
+                               const pp_policy* policy);
+
+/* TopN(n) as a standalone pass (truncation/builtin.rs:73-129): ties at the n-th magnitude are kept
+ * in the reference's canonical order. */
+pp_status pp_truncate_topn(pp_sum* sum, uint64_t n);
+
+/* expectation_product (readout/product_state.rs:84-101) in the product state whose qubit q is the
+ * +-1 eigenstate of the axis (axis_x, axis_z) bits, -1 where `negative` is set.  W words each. */
+pp_status pp_expectation_product(pp_sum* sum, const uint64_t* axis_x, const uint64_t* axis_z,
+                                 const uint64_t* negative, double* out);
+
+/* ---- measurement --------------------------------------------------------------------------- */
+
+/* While enabled, the rotation-join kernel of every layer is bracketed by hipEvents on the context's
+ * stream; pp_profile_read returns the launches and their total milliseconds since it was enabled
+ * (and the rows read and written by those launches), after a stream synchronization. */
+pp_status pp_profile_enable(pp_context* ctx, int enable);
+pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms, uint64_t* rows_in,
+                          uint64_t* rows_out);
+
+/* The reference's hash rows (Gf2Hash::new, pauli_sum/hash.rs:212-258), for checks: count*W words each. */
+int32_t pp_hash_rows(uint32_t num_qubits, uint64_t seed, uint32_t width, uint64_t* rows_x, uint64_t* rows_z);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -9,6 +9,7 @@
  * loop with the reference's run grouping (which decides the bucket count every
  * layer sees, and so the tie order of TopN and the canonical output order).
  */
+#define _GNU_SOURCE /* sincos */
 #include "pp_oracle.h"
 
 #include <math.h>
@@ -149,10 +150,13 @@ int ppo_rebucketed_bits(size_t len, int bits, size_t target, size_t min_buckets)
 
 /* ---- the rotation table (channel/rotation.rs:43-123) ---------------------------------------- */
 
+/* Rust's f64::sin_cos is sin and cos of one argument as LLVM intrinsics, which LLVM lowers to one call of
+ * glibc's sincos on x86_64 Linux; glibc's sincos and its separate sin and cos differ in the last bit on rare
+ * arguments, so the call is spelled out here rather than left to the compiler's own fusion. */
 void ppo_sin_cos(double theta, double* s, double* c) {
     const double snap = 1e-15;
-    double sv = sin(theta);
-    double cv = cos(theta);
+    double sv, cv;
+    sincos(theta, &sv, &cv);
     if (fabs(sv) < snap) {
         *s = 0.0;
         *c = copysign(1.0, cv);
@@ -242,6 +246,15 @@ void ppo_sum_free(ppo_sum* s) {
     if (!s) return;
     DISPATCH(s, free_buckets_w1_f64(s), free_buckets_w1_f32(s), free_buckets_w2_f64(s), free_buckets_w2_f32(s));
     free(s);
+}
+
+ppo_sum* ppo_sum_clone(const ppo_sum* src) {
+    ppo_sum* s = (ppo_sum*)malloc(sizeof(ppo_sum));
+    *s = *src;
+    s->buckets = NULL;
+    DISPATCH(src, clone_buckets_w1_f64(src, s), clone_buckets_w1_f32(src, s), clone_buckets_w2_f64(src, s),
+             clone_buckets_w2_f32(src, s));
+    return s;
 }
 
 size_t ppo_sum_len(const ppo_sum* s) { return s->len; }

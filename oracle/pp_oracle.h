@@ -75,6 +75,7 @@ typedef struct ppo_sum ppo_sum;
 ppo_sum* ppo_sum_new(int W, int kind, int num_qubits, size_t n, const uint64_t* x, const uint64_t* z,
                      const void* coeff, uint64_t seed);
 void ppo_sum_free(ppo_sum* s);
+ppo_sum* ppo_sum_clone(const ppo_sum* s);
 size_t ppo_sum_len(const ppo_sum* s);
 int ppo_sum_bits(const ppo_sum* s);
 /* Canonical order: bucket ascending, lex (x, z) within a bucket.  `bucket` may be NULL. */

@@ -57,6 +57,8 @@ def lib() -> ctypes.CDLL:
         L.ppo_sum_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, vp, vp, vp,
                                   ctypes.c_uint64]
         L.ppo_sum_free.argtypes = [vp]
+        L.ppo_sum_clone.restype = vp
+        L.ppo_sum_clone.argtypes = [vp]
         L.ppo_sum_len.restype = ctypes.c_size_t
         L.ppo_sum_len.argtypes = [vp]
         L.ppo_sum_bits.restype = ctypes.c_int
@@ -149,6 +151,12 @@ class OracleSum:
 
     def __len__(self):
         return lib().ppo_sum_len(self._h)
+
+    def clone(self) -> "OracleSum":
+        out = OracleSum.__new__(OracleSum)
+        out.width, out.kind, out.num_qubits = self.width, self.kind, self.num_qubits
+        out._h = lib().ppo_sum_clone(self._h)
+        return out
 
     @property
     def bits(self) -> int:

@@ -129,6 +129,23 @@ static void T(free_buckets)(ppo_sum* s) {
     s->buckets = NULL;
 }
 
+static void T(clone_buckets)(const ppo_sum* src, ppo_sum* dst) {
+    size_t nb = (size_t)1 << src->bits;
+    const T(bucket)* a = (const T(bucket)*)src->buckets;
+    T(bucket)* bk = (T(bucket)*)calloc(nb, sizeof(T(bucket)));
+    for (size_t b = 0; b < nb; ++b) {
+        T(bucket_reserve)(&bk[b], a[b].len ? a[b].len : 1);
+        memcpy(bk[b].k, a[b].k, (size_t)a[b].len * sizeof(T(key)));
+        memcpy(bk[b].c, a[b].c, (size_t)a[b].len * sizeof(REAL));
+        bk[b].len = a[b].len;
+        for (int w = 0; w < W; ++w) {
+            bk[b].sx[w] = a[b].sx[w];
+            bk[b].sz[w] = a[b].sz[w];
+        }
+    }
+    dst->buckets = bk;
+}
+
 /* ---- construction ---------------------------------------------------------------------- */
 
 typedef struct {

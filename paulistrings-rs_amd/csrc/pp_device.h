@@ -1,0 +1,222 @@
+// pp_device.h -- device-side building blocks of the MI355X rotation-layer engine.
+//
+// Keys are the symplectic Pauli encoding of the reference (crates/paulistrings/src/pauli_string.rs:15-20):
+// W 64-bit words of x bits and W of z bits.  Columns live in HBM word-major ("SoA per word"):
+// x[w * cap + i], z[w * cap + i], c[i], so every wave loads 512 contiguous bytes per word.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pp {
+
+constexpr int kBlock = 256;  // threads per workgroup: four wave64s
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxKeepNodes = 15;
+constexpr int kMaxHashRows = 31;
+
+enum KeepOp : int32_t { KEEP_TRUE = 0, KEEP_COEFF = 1, KEEP_WEIGHT = 2, KEEP_AND = 3, KEEP_OR = 4 };
+
+// A built-in policy's per-term filter as a postfix program (truncation/builtin.rs:19-59, :470-526; the
+// reference's device form is src/engine/gpu/truncation.rs:22-102).  eps2 = eps * eps is formed on the host
+// in f64, exactly as the reference forms it per term.
+struct KeepProgram {
+    int32_t n;
+    int32_t op[kMaxKeepNodes];
+    uint32_t k[kMaxKeepNodes];
+    uint32_t disabled[kMaxKeepNodes];  // CoefficientThreshold with eps < 0 keeps everything
+    double eps2[kMaxKeepNodes];
+};
+
+template <int W>
+struct Key {
+    uint64_t x[W];
+    uint64_t z[W];
+};
+
+template <int W>
+__device__ __forceinline__ Key<W> load_key(const uint64_t* __restrict__ x, const uint64_t* __restrict__ z,
+                                           uint64_t cap, uint64_t i) {
+    Key<W> k;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        k.x[w] = x[w * cap + i];
+        k.z[w] = z[w * cap + i];
+    }
+    return k;
+}
+
+template <int W>
+__device__ __forceinline__ void store_key(uint64_t* __restrict__ x, uint64_t* __restrict__ z, uint64_t cap,
+                                          uint64_t i, const Key<W>& k) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        __builtin_nontemporal_store(k.x[w], &x[w * cap + i]);
+        __builtin_nontemporal_store(k.z[w], &z[w * cap + i]);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_coeff(T* __restrict__ c, uint64_t i, T v) {
+    __builtin_nontemporal_store(v, &c[i]);
+}
+
+// commutes_with (pauli_string.rs:116-123): anticommute iff popcount(x.gz ^ z.gx) is odd.
+template <int W>
+__device__ __forceinline__ bool anticommutes(const Key<W>& k, const Key<W>& g) {
+    uint64_t f = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) f ^= (k.x[w] & g.z[w]) ^ (k.z[w] & g.x[w]);
+    return __popcll(f) & 1;
+}
+
+// The exponent of the phase i^d of k * g (pauli_string.rs:73-93), mod 4.
+template <int W>
+__device__ __forceinline__ uint32_t product_phase(const Key<W>& k, const Key<W>& g) {
+    uint32_t d = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const uint64_t a = k.x[w], b = k.z[w], c = g.x[w], e = g.z[w];
+        d += 2u * __popcll(b & c) + __popcll(a & b) + __popcll(c & e) - __popcll((a ^ c) & (b ^ e));
+    }
+    return d & 3u;
+}
+
+// The partner amplitude of an anticommuting k: i * sin * (k g) carries the real phase i^(1 + d),
+// a sign (channel/rotation.rs:57-65).
+template <int W, typename T>
+__device__ __forceinline__ T partner_amplitude(const Key<W>& k, const Key<W>& g, T sin_t) {
+    const uint32_t total = (1u + product_phase<W>(k, g)) & 3u;
+    return (total & 2u) ? -sin_t : sin_t;
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t weight(const Key<W>& k) {
+    uint32_t n = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) n += __popcll(k.x[w] | k.z[w]);
+    return n;
+}
+
+template <int W>
+__device__ __forceinline__ bool key_eq(const Key<W>& a, const Key<W>& b) {
+    uint64_t d = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) d |= (a.x[w] ^ b.x[w]) | (a.z[w] ^ b.z[w]);
+    return d == 0;
+}
+
+template <int W>
+__device__ __forceinline__ Key<W> key_xor(const Key<W>& a, const Key<W>& b) {
+    Key<W> r;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        r.x[w] = a.x[w] ^ b.x[w];
+        r.z[w] = a.z[w] ^ b.z[w];
+    }
+    return r;
+}
+
+// keep: nonzero and the policy's keep_term (rotation_layer/merge.rs:232-239).  The program is uniform
+// across the wave (a kernel argument), so the interpreter loop costs scalar branches only.
+template <int W, typename T>
+__device__ __forceinline__ bool survives(const KeepProgram& p, const Key<W>& k, T c) {
+    if (!(c != T(0))) return false;
+    if (p.n == 0) return true;
+    const double v = (double)c;
+    // The operand stack is a bit string (top = bit 0): a private array indexed by a loop counter would
+    // live in scratch memory.
+    uint32_t stack = 0;
+    for (int i = 0; i < p.n; ++i) {
+        const int32_t op = p.op[i];
+        if (op == KEEP_COEFF) {
+            stack = (stack << 1) | (uint32_t)(p.disabled[i] || (v * v > p.eps2[i]));
+        } else if (op == KEEP_WEIGHT) {
+            stack = (stack << 1) | (uint32_t)(weight<W>(k) <= p.k[i]);
+        } else if (op == KEEP_AND) {
+            const uint32_t r = stack & (stack >> 1) & 1u;
+            stack = ((stack >> 2) << 1) | r;
+        } else if (op == KEEP_OR) {
+            const uint32_t r = (stack | (stack >> 1)) & 1u;
+            stack = ((stack >> 2) << 1) | r;
+        } else {
+            stack = (stack << 1) | 1u;
+        }
+    }
+    return stack & 1u;
+}
+
+// Row parity of a GF(2) hash row: bit r of H.v (pauli_sum/hash.rs:170-179).
+template <int W>
+__device__ __forceinline__ uint32_t row_parity(const Key<W>& k, const uint64_t* __restrict__ rows, int r) {
+    uint64_t f = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) f ^= (k.x[w] & rows[(2 * r) * W + w]) ^ (k.z[w] & rows[(2 * r + 1) * W + w]);
+    return __popcll(f) & 1u;
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t bucket_of(const Key<W>& k, const uint64_t* __restrict__ rows, int bits) {
+    uint32_t b = 0;
+    for (int r = 0; r < bits; ++r) b |= row_parity<W>(k, rows, r) << r;
+    return b;
+}
+
+// A 32-bit mix of the key for the LDS join table; any function works, a good one keeps probes short.
+template <int W>
+__device__ __forceinline__ uint32_t key_hash(const Key<W>& k) {
+    uint32_t h = 0x9E3779B9u;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        uint32_t a = (uint32_t)k.x[w] ^ (uint32_t)(k.x[w] >> 32) * 0x85EBCA77u;
+        uint32_t b = (uint32_t)k.z[w] ^ (uint32_t)(k.z[w] >> 32) * 0xC2B2AE3Du;
+        h = (h ^ a) * 0x27D4EB2Fu;
+        h = (h ^ (h >> 15)) ^ b;
+        h *= 0x165667B1u;
+    }
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+__device__ __forceinline__ uint32_t wave_id() { return threadIdx.x >> 6; }
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const uint32_t l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Ordered compaction of NF one-bit flags across the workgroup: ballot + popcount within a wave64, the four
+// wave totals through LDS.  `off[i]` is the thread's rank among set flags i, `tot[i]` the block's count.
+// `scratch` holds NF * kWaves words.  Two barriers; every thread of the block must call it.
+template <int NF>
+__device__ __forceinline__ void block_flag_scan(const bool (&f)[NF], uint32_t (&off)[NF], uint32_t (&tot)[NF],
+                                                uint32_t* scratch) {
+    const uint32_t lane = lane_id(), wave = wave_id();
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+        const uint64_t m = __ballot(f[i]);
+        off[i] = __popcll(m & lt);
+        if (lane == 0) scratch[i * kWaves + wave] = __popcll(m);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+        uint32_t before = 0, all = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = scratch[i * kWaves + w];
+            before += (uint32_t)w < wave ? c : 0u;
+            all += c;
+        }
+        off[i] += before;
+        tot[i] = all;
+    }
+    __syncthreads();
+}
+
+}  // namespace pp

@@ -1,0 +1,1141 @@
+// pp_host.hip -- the host side of the MI355X rotation-layer engine and its C ABI (include/pp_hip.h).
+//
+// A pp_sum is resident in HBM.  Its device bucketing (GF(2) hash rows of the reference, at a bucket count chosen
+// here for LDS-sized pairs) is independent of the reference's bucket count, which this file tracks alongside
+// exactly as the reference's layer loop moves it (engine/driver.rs:186-258, :319-592), because that count fixes
+// the canonical output order and the order in which TopN keeps tied terms.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/pp_hip.h"
+#include "pp_kernels.h"
+#include "pp_topn.h"
+
+namespace {
+
+using namespace pp;
+
+constexpr int kIndexBits = 31;                    // pauli_sum/hash.rs:7
+constexpr size_t kMinTermsPerTask = 64;           // pauli_sum/storage.rs:48
+constexpr size_t kCoarsenBelow = 8;               // pauli_sum/storage.rs:44
+constexpr uint32_t kRedrawAttempts = 4096;        // pauli_sum/hash.rs:18
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;  // rng.rs:5
+constexpr size_t kRunLayers = 8;                  // engine/bucketed/rotation_layer.rs:508
+constexpr int kRunDimension = 3;                  // engine/bucketed/rotation_layer.rs:511
+constexpr uint32_t kMaxPairRows = 1024;           // LDS rows of one join workgroup
+
+uint32_t device_bucket_target() {
+    static uint32_t v = [] {
+        const char* e = std::getenv("PP_DEVICE_BUCKET_LEN");
+        long t = e ? std::strtol(e, nullptr, 10) : 0;
+        return (uint32_t)(t >= 16 && t <= 512 ? t : 128);
+    }();
+    return v;
+}
+
+// ---- the reference's hash rows and bucket-count policy ---------------------------------------------------------
+
+uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint64_t live_mask(uint32_t num_qubits, uint32_t word) {
+    const uint32_t first = 64 * word;
+    if (num_qubits >= first + 64) return ~0ull;
+    if (num_qubits <= first) return 0;
+    return (1ull << (num_qubits - first)) - 1ull;
+}
+
+// Gf2Hash::new rows: splitmix64 draws, redrawn while dependent (pauli_sum/hash.rs:22-130).  Returns the row
+// count; rows_x/rows_z receive count * W words.
+int hash_rows(uint32_t num_qubits, uint64_t seed, uint32_t W, uint64_t* rows_x, uint64_t* rows_z) {
+    const int rows = (int)std::min<uint32_t>(2 * num_qubits, kIndexBits);
+    std::vector<std::vector<uint64_t>> basis;  // reduced rows, each with a distinct leading column
+    std::vector<int> leads;
+    auto leading = [&](const std::vector<uint64_t>& r) {
+        for (size_t w = 0; w < r.size(); ++w)
+            if (r[w]) return (int)(64 * w + __builtin_clzll(r[w]));
+        return -1;
+    };
+    auto insert = [&](std::vector<uint64_t> r) {
+        for (;;) {
+            const int l = leading(r);
+            if (l < 0) return false;
+            auto it = std::find(leads.begin(), leads.end(), l);
+            if (it == leads.end()) {
+                leads.push_back(l);
+                basis.push_back(r);
+                return true;
+            }
+            const auto& p = basis[it - leads.begin()];
+            for (size_t w = 0; w < r.size(); ++w) r[w] ^= p[w];
+        }
+    };
+    for (int r = 0; r < rows; ++r) {
+        std::vector<uint64_t> key(2 * W);
+        for (uint32_t attempt = 0;; ++attempt) {
+            bool zero = true;
+            for (uint32_t w = 0; w < W; ++w) {
+                const uint64_t m = live_mask(num_qubits, w);
+                for (uint32_t half = 0; half < 2; ++half) {
+                    const uint64_t pos = ((uint64_t)attempt << 32) | ((uint64_t)r << 16) | ((uint64_t)w << 1) | half;
+                    key[half * W + w] = mix64(seed + kGamma * (pos + 1)) & m;
+                    zero = zero && key[half * W + w] == 0;
+                }
+            }
+            const bool accepted = (int)basis.size() < rows ? insert(key) : !zero;
+            if (accepted || attempt == kRedrawAttempts) break;
+        }
+        for (uint32_t w = 0; w < W; ++w) {
+            rows_x[r * W + w] = key[w];
+            rows_z[r * W + w] = key[W + w];
+        }
+    }
+    return rows;
+}
+
+int desired_bits(size_t len, size_t target, size_t min_buckets) {
+    int floor_bits = 0;
+    if (len >= min_buckets * kMinTermsPerTask)
+        while (((size_t)1 << floor_bits) < min_buckets && floor_bits < kIndexBits) ++floor_bits;
+    int b = floor_bits;
+    while (b < kIndexBits && len > target * ((size_t)1 << b)) ++b;
+    return b;
+}
+
+int rebucketed_bits(size_t len, int bits, size_t target, size_t min_buckets) {
+    const int want = desired_bits(len, target, min_buckets);
+    const bool sparse = len * kCoarsenBelow < target * ((size_t)1 << bits);
+    return (want > bits || sparse) ? want : bits;
+}
+
+// sin_cos with quarter turns snapped to exact 0 / +-1 (channel/rotation.rs:70-78), from the host libm.  Rust's
+// f64::sin_cos reaches glibc as one sincos call on x86_64 Linux (LLVM fuses the sin and cos intrinsics of one
+// argument); glibc's sincos and its separate sin and cos differ in the last bit on rare arguments, so sincos is
+// called by name.
+void snapped_sin_cos(double theta, double* s, double* c) {
+    const double snap = 1e-15;
+    double sv, cv;
+    ::sincos(theta, &sv, &cv);
+    if (std::fabs(sv) < snap) {
+        *s = 0.0;
+        *c = std::copysign(1.0, cv);
+    } else if (std::fabs(cv) < snap) {
+        *s = std::copysign(1.0, sv);
+        *c = 0.0;
+    } else {
+        *s = sv;
+        *c = cv;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------------
+// Context and sum
+// ---------------------------------------------------------------------------------------------------------------
+
+struct pp_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    uint64_t prof_rows_in = 0;
+    unsigned long long* d_prof = nullptr;  // [0]: output rows of the profiled joins
+};
+
+struct Columns {
+    uint64_t* x = nullptr;
+    uint64_t* z = nullptr;
+    void* c = nullptr;
+    uint64_t cap = 0;
+};
+
+struct BucketArrays {
+    uint64_t* start = nullptr;
+    uint32_t* len = nullptr;
+    uint64_t cap = 0;
+};
+
+struct pp_sum {
+    pp_context* ctx = nullptr;
+    uint32_t num_qubits = 0;
+    uint32_t W = 1;
+    pp_coeff_kind kind = PP_COEFF_F64;
+    uint64_t seed = 0;
+    int max_bits = 0;
+    std::vector<uint64_t> rows;  // [row][x|z][W]
+    uint64_t* d_rows = nullptr;
+    Columns col[2];
+    BucketArrays bk[2];
+    int cur = 0;
+    int device_bits = 0;
+    int reference_bits = 0;
+    uint64_t len = 0;
+    uint32_t max_bucket = 0;
+    bool stale = false;  // len and max_bucket await the last layer's counters (pp_apply_rotation_to)
+    unsigned long long* d_stats = nullptr;
+    unsigned long long* h_stats = nullptr;  // pinned
+    pp::TopnState* d_topn = nullptr;
+    unsigned long long* d_hist = nullptr;
+    size_t coeff_bytes() const { return kind == PP_COEFF_F64 ? 8 : 4; }
+};
+
+namespace {
+
+pp_status fail(pp_context* ctx, pp_status s, const std::string& msg) {
+    if (ctx) ctx->error = msg;
+    return s;
+}
+
+#define PP_HIP(ctx, call)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail((ctx), PP_ERR_DRIVER, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+pp_status dev_alloc(pp_context* ctx, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 64;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, PP_ERR_OUT_OF_MEMORY,
+                    "device " + std::to_string(ctx->device) + ": out of memory allocating " + std::to_string(bytes) +
+                        " bytes");
+    }
+    return PP_OK;
+}
+
+void free_columns(Columns& c) {
+    if (c.x) (void)hipFree(c.x);
+    if (c.z) (void)hipFree(c.z);
+    if (c.c) (void)hipFree(c.c);
+    c = Columns{};
+}
+
+void free_buckets(BucketArrays& b) {
+    if (b.start) (void)hipFree(b.start);
+    if (b.len) (void)hipFree(b.len);
+    b = BucketArrays{};
+}
+
+pp_status ensure_columns(pp_sum* s, int which, uint64_t need) {
+    Columns& c = s->col[which];
+    if (c.cap >= need && c.x) return PP_OK;
+    free_columns(c);
+    const uint64_t cap = std::max<uint64_t>(need + need / 4, 1024);
+    pp_status st;
+    if ((st = dev_alloc(s->ctx, (void**)&c.x, cap * s->W * 8)) != PP_OK) return st;
+    if ((st = dev_alloc(s->ctx, (void**)&c.z, cap * s->W * 8)) != PP_OK) return st;
+    if ((st = dev_alloc(s->ctx, &c.c, cap * s->coeff_bytes())) != PP_OK) return st;
+    c.cap = cap;
+    return PP_OK;
+}
+
+pp_status ensure_buckets(pp_sum* s, int which, uint64_t need) {
+    BucketArrays& b = s->bk[which];
+    if (b.cap >= need && b.start) return PP_OK;
+    free_buckets(b);
+    const uint64_t cap = std::max<uint64_t>(need, 256);
+    pp_status st;
+    if ((st = dev_alloc(s->ctx, (void**)&b.start, cap * 8)) != PP_OK) return st;
+    if ((st = dev_alloc(s->ctx, (void**)&b.len, cap * 4)) != PP_OK) return st;
+    b.cap = cap;
+    return PP_OK;
+}
+
+// Dispatch on (W, coefficient type).
+template <typename F>
+auto dispatch(const pp_sum* s, F&& f) {
+    if (s->W == 1) {
+        if (s->kind == PP_COEFF_F64) return f(std::integral_constant<int, 1>{}, double{});
+        return f(std::integral_constant<int, 1>{}, float{});
+    }
+    if (s->kind == PP_COEFF_F64) return f(std::integral_constant<int, 2>{}, double{});
+    return f(std::integral_constant<int, 2>{}, float{});
+}
+
+uint32_t bucket_of_host(const pp_sum* s, const uint64_t* gx, const uint64_t* gz, int bits) {
+    uint32_t b = 0;
+    for (int r = 0; r < bits; ++r) {
+        uint64_t f = 0;
+        for (uint32_t w = 0; w < s->W; ++w)
+            f ^= (gx[w] & s->rows[(2 * r) * s->W + w]) ^ (gz[w] & s->rows[(2 * r + 1) * s->W + w]);
+        b |= (uint32_t)(__builtin_popcountll(f) & 1) << r;
+    }
+    return b;
+}
+
+pp_status read_stats(pp_sum* s) {
+    pp_context* ctx = s->ctx;
+    PP_HIP(ctx, hipMemcpyAsync(s->h_stats, s->d_stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PP_OK;
+}
+
+pp_status clear_stats(pp_sum* s) {
+    PP_HIP(s->ctx, hipMemsetAsync(s->d_stats, 0, 4 * sizeof(unsigned long long), s->ctx->stream));
+    return PP_OK;
+}
+
+// ---- device rebucketing ------------------------------------------------------------------------------------------
+
+pp_status refine_once(pp_sum* s) {
+    const int in = s->cur, out = 1 - s->cur;
+    const uint32_t B = 1u << s->device_bits;
+    pp_status st;
+    if ((st = ensure_columns(s, out, s->col[in].cap)) != PP_OK) return st;
+    if ((st = ensure_buckets(s, out, 2ull * B)) != PP_OK) return st;
+    if ((st = clear_stats(s)) != PP_OK) return st;
+    hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
+        constexpr int W = decltype(Wc)::value;
+        using T = decltype(Tv);
+        RefineArgs<W, T> a{};
+        a.x = s->col[in].x;
+        a.z = s->col[in].z;
+        a.c = (const T*)s->col[in].c;
+        a.cap = s->col[in].cap;
+        a.start = s->bk[in].start;
+        a.len = s->bk[in].len;
+        a.ox = s->col[out].x;
+        a.oz = s->col[out].z;
+        a.oc = (T*)s->col[out].c;
+        a.ocap = s->col[out].cap;
+        a.ostart = s->bk[out].start;
+        a.olen = s->bk[out].len;
+        a.num_buckets = B;
+        a.row = s->device_bits;
+        a.rows = s->d_rows;
+        a.stats = s->d_stats;
+        return launch_refine<W, T>(s->ctx->stream, a);
+    });
+    PP_HIP(s->ctx, e);
+    if ((st = read_stats(s)) != PP_OK) return st;
+    s->max_bucket = (uint32_t)s->h_stats[1];
+    s->device_bits += 1;
+    s->cur = out;
+    return PP_OK;
+}
+
+pp_status coarsen_to(pp_sum* s, int bits) {
+    const int in = s->cur, out = 1 - s->cur;
+    const uint32_t B = 1u << s->device_bits, CB = 1u << bits;
+    pp_status st;
+    if ((st = ensure_columns(s, out, std::max<uint64_t>(s->len, 1))) != PP_OK) return st;
+    if ((st = ensure_buckets(s, out, CB)) != PP_OK) return st;
+    if ((st = clear_stats(s)) != PP_OK) return st;
+    PP_HIP(s->ctx, launch_bucket_offsets(s->ctx->stream, s->bk[in].len, B, 2, 0, CB, B / CB, s->bk[out].start,
+                                         s->d_stats + 3));
+    hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
+        constexpr int W = decltype(Wc)::value;
+        using T = decltype(Tv);
+        CoarsenArgs<W, T> a{};
+        a.x = s->col[in].x;
+        a.z = s->col[in].z;
+        a.c = (const T*)s->col[in].c;
+        a.cap = s->col[in].cap;
+        a.start = s->bk[in].start;
+        a.len = s->bk[in].len;
+        a.ox = s->col[out].x;
+        a.oz = s->col[out].z;
+        a.oc = (T*)s->col[out].c;
+        a.ocap = s->col[out].cap;
+        a.ostart = s->bk[out].start;
+        a.olen = s->bk[out].len;
+        a.num_buckets = B;
+        a.coarse_buckets = CB;
+        a.stats = s->d_stats;
+        return launch_coarsen<W, T>(s->ctx->stream, a);
+    });
+    PP_HIP(s->ctx, e);
+    if ((st = read_stats(s)) != PP_OK) return st;
+    s->max_bucket = (uint32_t)s->h_stats[1];
+    s->device_bits = bits;
+    s->cur = out;
+    return PP_OK;
+}
+
+// The device's own bucket count: about device_bucket_target() rows per bucket, with a factor-8 hysteresis, and
+// fine enough that the largest pair fits one workgroup's LDS.
+pp_status settle_device_bits(pp_sum* s) {
+    const uint64_t target = device_bucket_target();
+    int want = 0;
+    while (want < s->max_bits && s->len > target * (1ull << want)) ++want;
+    pp_status st;
+    if (want > s->device_bits) {
+        while (s->device_bits < want)
+            if ((st = refine_once(s)) != PP_OK) return st;
+    } else if (s->len * 8 < target * (1ull << s->device_bits) && want < s->device_bits) {
+        if ((st = coarsen_to(s, want)) != PP_OK) return st;
+    }
+    while (2ull * s->max_bucket > kMaxPairRows) {
+        if (s->device_bits >= s->max_bits)
+            return fail(s->ctx, PP_ERR_UNSUPPORTED,
+                        "a bucket of " + std::to_string(s->max_bucket) + " terms cannot be split further");
+        if ((st = refine_once(s)) != PP_OK) return st;
+    }
+    return PP_OK;
+}
+
+// ---- one prepared channel ------------------------------------------------------------------------------------------
+
+struct Prepared {
+    double sin = 0, cos = 1;
+    int weight = 0;
+    bool local = true;            // Prepared::Local (support <= 2) rather than Prepared::Rotation
+    bool key_preserving = false;  // only the identity delta survives
+    bool clifford = false;        // the identity residue is dropped (rotation_layer.rs:117-121)
+};
+
+Prepared prepare(const pp_sum* s, const pp_channel& ch, bool adjoint) {
+    Prepared p;
+    snapped_sin_cos(adjoint ? -ch.theta : ch.theta, &p.sin, &p.cos);
+    for (uint32_t w = 0; w < s->W; ++w) p.weight += __builtin_popcountll(ch.gx[w] | ch.gz[w]);
+    p.local = p.weight <= 2;
+    p.key_preserving = p.weight == 0 || p.sin == 0.0;
+    const double a0 = s->kind == PP_COEFF_F64 ? p.cos : (double)(float)p.cos;
+    const double a1 = s->kind == PP_COEFF_F64 ? p.sin : (double)(float)p.sin;
+    const double eps = 2.220446049250313e-16;
+    p.clifford = !p.key_preserving && a0 * a0 <= eps * eps * (a1 * a1);
+    return p;
+}
+
+size_t fanout(const Prepared& p) {  // max_fanout, engine/bucketed.rs:628-641
+    if (!p.local) return 2;
+    return (p.key_preserving || p.clifford) ? 1 : 2;
+}
+
+bool is_permutation_step(const Prepared& p) {  // PermutationStep::of, permutation_run.rs:53-121
+    return p.local && p.weight > 0 && (p.key_preserving || p.clifford);
+}
+
+bool is_rotation_table(const Prepared& p) { return p.local && !p.key_preserving; }  // RotationLayer::of
+
+int span_dimension(std::vector<uint32_t> v) {
+    int rank = 0;
+    for (int bit = 31; bit >= 0; --bit) {
+        auto it = std::find_if(v.begin(), v.end(), [&](uint32_t d) { return (d >> bit) & 1u; });
+        if (it == v.end()) continue;
+        const uint32_t p = *it;
+        v.erase(it);
+        for (auto& d : v)
+            if ((d >> bit) & 1u) d ^= p;
+        ++rank;
+    }
+    return rank;
+}
+
+KeepProgram lower_policy(const pp_policy* pol) {
+    KeepProgram k{};
+    if (!pol) return k;
+    k.n = std::min<int32_t>(pol->n_nodes, kMaxKeepNodes);
+    for (int i = 0; i < k.n; ++i) {
+        k.op[i] = pol->nodes[i].op;
+        k.k[i] = pol->nodes[i].k;
+        k.disabled[i] = pol->nodes[i].eps < 0.0;
+        k.eps2[i] = pol->nodes[i].eps * pol->nodes[i].eps;
+    }
+    return k;
+}
+
+bool valid_policy(const pp_policy* pol) {
+    if (!pol) return true;
+    if (pol->n_nodes < 0 || pol->n_nodes > 15) return false;
+    int depth = 0;
+    for (int i = 0; i < pol->n_nodes; ++i) {
+        const int op = pol->nodes[i].op;
+        if (op == PP_KEEP_AND || op == PP_KEEP_OR) {
+            if (depth < 2) return false;
+            --depth;
+        } else if (op == PP_KEEP_TRUE || op == PP_KEEP_COEFF || op == PP_KEEP_WEIGHT) {
+            ++depth;
+        } else {
+            return false;
+        }
+    }
+    return pol->n_nodes == 0 || depth == 1;
+}
+
+// ---- one layer on the device ---------------------------------------------------------------------------------------
+
+pp_status refresh(pp_sum* s);
+
+// The rotation join from `src`'s current columns into `dst`'s column set `out` (dst may be src, with out its
+// spare set).  Asynchronous: the counters land in dst->d_stats.
+pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, const Prepared& p,
+                      const KeepProgram& keep) {
+    pp_context* ctx = src->ctx;
+    pp_status st;
+    const int in = src->cur;
+    const uint32_t B = 1u << src->device_bits;
+    if ((st = ensure_columns(dst, out, 2 * src->len + 1)) != PP_OK) return st;
+    if ((st = ensure_buckets(dst, out, B)) != PP_OK) return st;
+    if ((st = clear_stats(dst)) != PP_OK) return st;
+    const uint32_t delta = bucket_of_host(src, ch.gx, ch.gz, src->device_bits);
+    PP_HIP(ctx, launch_bucket_offsets(ctx->stream, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start,
+                                      dst->d_stats + 3));
+    uint32_t lcap = delta ? 2u * src->max_bucket : src->max_bucket;
+    lcap = std::max<uint32_t>(64, (lcap + 63) & ~63u);
+    if (lcap > kMaxPairRows) lcap = kMaxPairRows;
+    uint32_t tcap_bits = 1;
+    while ((1u << tcap_bits) < 2u * lcap) ++tcap_bits;
+    const uint32_t cosets = delta ? B / 2 : B;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (ctx->profiling) {
+        PP_HIP(ctx, hipEventCreate(&ev0));
+        PP_HIP(ctx, hipEventCreate(&ev1));
+        PP_HIP(ctx, hipEventRecord(ev0, ctx->stream));
+    }
+    hipError_t e = dispatch(src, [&](auto Wc, auto Tv) {
+        constexpr int W = decltype(Wc)::value;
+        using T = decltype(Tv);
+        JoinArgs<W, T> a{};
+        a.x = src->col[in].x;
+        a.z = src->col[in].z;
+        a.c = (const T*)src->col[in].c;
+        a.cap = src->col[in].cap;
+        a.start = src->bk[in].start;
+        a.len = src->bk[in].len;
+        a.ox = dst->col[out].x;
+        a.oz = dst->col[out].z;
+        a.oc = (T*)dst->col[out].c;
+        a.ocap = dst->col[out].cap;
+        a.ostart = dst->bk[out].start;
+        a.olen = dst->bk[out].len;
+        a.delta = delta;
+        a.hb = delta ? 31u - (uint32_t)__builtin_clz(delta) : 0u;
+        a.lcap = lcap;
+        a.tcap_bits = tcap_bits;
+        for (uint32_t w = 0; w < W; ++w) {
+            a.gx[w] = ch.gx[w];
+            a.gz[w] = ch.gz[w];
+        }
+        a.cos_t = (T)p.cos;
+        a.sin_t = (T)p.sin;
+        a.clifford = p.clifford ? 1 : 0;
+        a.keep = keep;
+        a.stats = dst->d_stats;
+        a.prof = ctx->profiling ? ctx->d_prof : nullptr;
+        return launch_rotation_join<W, T>(ctx->stream, a, cosets);
+    });
+    PP_HIP(ctx, e);
+    if (ctx->profiling) {
+        PP_HIP(ctx, hipEventRecord(ev1, ctx->stream));
+        ctx->events.push_back({ev0, ev1});
+        ctx->prof_rows_in += src->len;
+    }
+    return PP_OK;
+}
+
+pp_status device_layer(pp_sum* s, const pp_channel& ch, const Prepared& p, const KeepProgram& keep) {
+    pp_context* ctx = s->ctx;
+    pp_status st;
+    if ((st = settle_device_bits(s)) != PP_OK) return st;
+    const uint32_t B = 1u << s->device_bits;
+    if (p.key_preserving) {
+        if ((st = clear_stats(s)) != PP_OK) return st;
+        hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
+            constexpr int W = decltype(Wc)::value;
+            using T = decltype(Tv);
+            RescaleArgs<W, T> a{};
+            a.x = s->col[s->cur].x;
+            a.z = s->col[s->cur].z;
+            a.c = (T*)s->col[s->cur].c;
+            a.cap = s->col[s->cur].cap;
+            a.start = s->bk[s->cur].start;
+            a.len = s->bk[s->cur].len;
+            for (uint32_t w = 0; w < W; ++w) {
+                a.gx[w] = ch.gx[w];
+                a.gz[w] = ch.gz[w];
+            }
+            a.a0 = (T)p.cos;
+            a.keep = keep;
+            a.stats = s->d_stats;
+            return launch_rescale<W, T>(ctx->stream, a, B);
+        });
+        PP_HIP(ctx, e);
+        if ((st = read_stats(s)) != PP_OK) return st;
+        s->len = s->h_stats[0];
+        s->max_bucket = (uint32_t)s->h_stats[1];
+        return PP_OK;
+    }
+    const int out = 1 - s->cur;
+    if ((st = launch_join(s, s, out, ch, p, keep)) != PP_OK) return st;
+    if ((st = read_stats(s)) != PP_OK) return st;
+    if (s->h_stats[2]) return fail(ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded its LDS rows");
+    s->len = s->h_stats[0];
+    s->max_bucket = (uint32_t)s->h_stats[1];
+    s->cur = out;
+    return PP_OK;
+}
+
+// Bring len and max_bucket up to date after an asynchronous out-of-place layer.
+pp_status refresh(pp_sum* s) {
+    if (!s->stale) return PP_OK;
+    pp_status st;
+    if ((st = read_stats(s)) != PP_OK) return st;
+    s->stale = false;
+    if (s->h_stats[2]) return fail(s->ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded its LDS rows");
+    s->len = s->h_stats[0];
+    s->max_bucket = (uint32_t)s->h_stats[1];
+    return PP_OK;
+}
+
+pp_status apply_one(pp_sum* s, const pp_channel& ch, const Prepared& p, const KeepProgram& keep) {
+    return device_layer(s, ch, p, keep);
+}
+
+pp_status topn_device(pp_sum* s, uint64_t n);  // pp_topn_impl.inc
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------------------------
+
+extern "C" {
+
+pp_status pp_context_create(int device, pp_context** out) {
+    if (!out) return PP_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return PP_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= count) return PP_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return PP_ERR_DRIVER;
+    auto* ctx = new (std::nothrow) pp_context();
+    if (!ctx) return PP_ERR_OUT_OF_MEMORY;
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return PP_ERR_DRIVER;
+    }
+    *out = ctx;
+    return PP_OK;
+}
+
+void pp_context_destroy(pp_context* ctx) {
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    (void)hipStreamDestroy(ctx->stream);
+    if (ctx->d_prof) (void)hipFree(ctx->d_prof);
+    delete ctx;
+}
+
+const char* pp_last_error(const pp_context* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
+
+pp_status pp_sync(pp_context* ctx) {
+    if (!ctx) return PP_ERR_INVALID_ARGUMENT;
+    PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PP_OK;
+}
+
+pp_status pp_context_stream(pp_context* ctx, void** stream) {
+    if (!ctx || !stream) return PP_ERR_INVALID_ARGUMENT;
+    *stream = (void*)ctx->stream;
+    return PP_OK;
+}
+
+int32_t pp_hash_rows(uint32_t num_qubits, uint64_t seed, uint32_t width, uint64_t* rows_x, uint64_t* rows_z) {
+    if (width < 1 || width > 4 || num_qubits > 64 * width) return -1;
+    return hash_rows(num_qubits, seed, width, rows_x, rows_z);
+}
+
+pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp_coeff_kind kind, uint64_t n,
+                        const uint64_t* x, const uint64_t* z, const void* coeff, uint64_t hash_seed,
+                        pp_sum** out) {
+    if (!ctx || !out) return PP_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (width != 1 && width != 2) return fail(ctx, PP_ERR_UNSUPPORTED, "width must be 1 or 2");
+    if (num_qubits > 64 * width) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "num_qubits exceeds 64 * width");
+    if (kind != PP_COEFF_F64 && kind != PP_COEFF_F32) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "coefficient kind");
+    if (n && (!x || !z || !coeff)) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "null term arrays");
+    PP_HIP(ctx, hipSetDevice(ctx->device));
+    auto* s = new (std::nothrow) pp_sum();
+    if (!s) return fail(ctx, PP_ERR_OUT_OF_MEMORY, "host allocation");
+    s->ctx = ctx;
+    s->num_qubits = num_qubits;
+    s->W = width;
+    s->kind = kind;
+    s->seed = hash_seed;
+    std::vector<uint64_t> rx(kIndexBits * width), rz(kIndexBits * width);
+    s->max_bits = hash_rows(num_qubits, hash_seed, width, rx.data(), rz.data());
+    s->rows.assign((size_t)kMaxHashRows * 2 * width, 0);
+    for (int r = 0; r < s->max_bits; ++r)
+        for (uint32_t w = 0; w < width; ++w) {
+            s->rows[(2 * r) * width + w] = rx[r * width + w];
+            s->rows[(2 * r + 1) * width + w] = rz[r * width + w];
+        }
+    auto bail = [&](pp_status st) {
+        pp_sum_free(s);
+        return st;
+    };
+    pp_status st;
+    if ((st = dev_alloc(ctx, (void**)&s->d_rows, s->rows.size() * 8)) != PP_OK) return bail(st);
+    if ((st = dev_alloc(ctx, (void**)&s->d_stats, 4 * sizeof(unsigned long long))) != PP_OK) return bail(st);
+    if (hipHostMalloc((void**)&s->h_stats, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned host allocation"));
+    if (hipMemcpy(s->d_rows, s->rows.data(), s->rows.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(ctx, PP_ERR_DRIVER, "hash rows upload"));
+
+    // Keep nonzero terms, bucket them at the device bucket count (counting sort on the host), transpose to
+    // word-major columns.
+    const size_t cb = s->coeff_bytes();
+    std::vector<uint64_t> keep_idx;
+    keep_idx.reserve(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const bool nz = kind == PP_COEFF_F64 ? ((const double*)coeff)[i] != 0.0 : ((const float*)coeff)[i] != 0.0f;
+        if (nz) keep_idx.push_back(i);
+    }
+    const uint64_t m = keep_idx.size();
+    s->len = m;
+    s->reference_bits = std::min(desired_bits(m, 1024, 128), s->max_bits);
+    const uint64_t target = device_bucket_target();
+    int dbits = 0;
+    while (dbits < s->max_bits && m > target * (1ull << dbits)) ++dbits;
+    s->device_bits = dbits;
+    const uint32_t B = 1u << dbits;
+    std::vector<uint32_t> bucket(m), counts(B, 0);
+    for (uint64_t t = 0; t < m; ++t) {
+        const uint64_t i = keep_idx[t];
+        bucket[t] = bucket_of_host(s, x + i * width, z + i * width, dbits);
+        counts[bucket[t]]++;
+    }
+    std::vector<uint64_t> start(B);
+    std::vector<uint32_t> lens(B);
+    uint64_t acc = 0;
+    uint32_t maxb = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+        start[b] = acc;
+        lens[b] = counts[b];
+        acc += counts[b];
+        maxb = std::max(maxb, counts[b]);
+    }
+    s->max_bucket = maxb;
+    if ((st = ensure_columns(s, 0, std::max<uint64_t>(m, 1))) != PP_OK) return bail(st);
+    if ((st = ensure_buckets(s, 0, B)) != PP_OK) return bail(st);
+    const uint64_t cap = s->col[0].cap;
+    std::vector<uint64_t> hx((size_t)width * cap, 0), hz((size_t)width * cap, 0);
+    std::vector<unsigned char> hc(cap * cb, 0);
+    std::vector<uint64_t> cursor(start);
+    for (uint64_t t = 0; t < m; ++t) {
+        const uint64_t i = keep_idx[t];
+        const uint64_t pos = cursor[bucket[t]]++;
+        for (uint32_t w = 0; w < width; ++w) {
+            hx[w * cap + pos] = x[i * width + w];
+            hz[w * cap + pos] = z[i * width + w];
+        }
+        std::memcpy(&hc[pos * cb], (const unsigned char*)coeff + i * cb, cb);
+    }
+    if (hipMemcpy(s->col[0].x, hx.data(), hx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->col[0].z, hz.data(), hz.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->col[0].c, hc.data(), hc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->bk[0].start, start.data(), B * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->bk[0].len, lens.data(), B * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(ctx, PP_ERR_DRIVER, "upload copy failed"));
+    s->cur = 0;
+    *out = s;
+    return PP_OK;
+}
+
+pp_status pp_sum_free(pp_sum* s) {
+    if (!s) return PP_OK;
+    if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+    for (int i = 0; i < 2; ++i) {
+        free_columns(s->col[i]);
+        free_buckets(s->bk[i]);
+    }
+    if (s->d_rows) (void)hipFree(s->d_rows);
+    if (s->d_stats) (void)hipFree(s->d_stats);
+    if (s->h_stats) (void)hipHostFree(s->h_stats);
+    if (s->d_topn) (void)hipFree(s->d_topn);
+    if (s->d_hist) (void)hipFree(s->d_hist);
+    delete s;
+    return PP_OK;
+}
+
+pp_status pp_sum_clone(const pp_sum* src_c, pp_sum** out) {
+    if (!src_c || !out) return PP_ERR_INVALID_ARGUMENT;
+    pp_sum* src = const_cast<pp_sum*>(src_c);
+    pp_context* ctx = src->ctx;
+    pp_status rs = refresh(src);
+    if (rs != PP_OK) return rs;
+    auto* s = new (std::nothrow) pp_sum();
+    if (!s) return fail(ctx, PP_ERR_OUT_OF_MEMORY, "host allocation");
+    s->ctx = ctx;
+    s->num_qubits = src->num_qubits;
+    s->W = src->W;
+    s->kind = src->kind;
+    s->seed = src->seed;
+    s->max_bits = src->max_bits;
+    s->rows = src->rows;
+    s->device_bits = src->device_bits;
+    s->reference_bits = src->reference_bits;
+    s->len = src->len;
+    s->max_bucket = src->max_bucket;
+    auto bail = [&](pp_status st) {
+        pp_sum_free(s);
+        return st;
+    };
+    pp_status st;
+    if ((st = dev_alloc(ctx, (void**)&s->d_rows, s->rows.size() * 8)) != PP_OK) return bail(st);
+    if ((st = dev_alloc(ctx, (void**)&s->d_stats, 4 * sizeof(unsigned long long))) != PP_OK) return bail(st);
+    if (hipHostMalloc((void**)&s->h_stats, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned host allocation"));
+    const Columns& sc = src->col[src->cur];
+    const BucketArrays& sb = src->bk[src->cur];
+    if ((st = ensure_columns(s, 0, sc.cap)) != PP_OK) return bail(st);
+    if ((st = ensure_buckets(s, 0, 1ull << src->device_bits)) != PP_OK) return bail(st);
+    const uint64_t B = 1ull << src->device_bits;
+    const uint64_t cap = s->col[0].cap;
+    hipStream_t stream = ctx->stream;
+    // word-major columns: copy each word plane at the new capacity stride
+    for (uint32_t w = 0; w < s->W; ++w) {
+        if (hipMemcpyAsync(s->col[0].x + w * cap, sc.x + w * sc.cap, sc.cap * 8, hipMemcpyDeviceToDevice, stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(s->col[0].z + w * cap, sc.z + w * sc.cap, sc.cap * 8, hipMemcpyDeviceToDevice, stream) !=
+                hipSuccess)
+            return bail(fail(ctx, PP_ERR_DRIVER, "clone copy failed"));
+    }
+    if (hipMemcpyAsync(s->col[0].c, sc.c, sc.cap * s->coeff_bytes(), hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(s->bk[0].start, sb.start, B * 8, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(s->bk[0].len, sb.len, B * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(s->d_rows, s->rows.data(), s->rows.size() * 8, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return bail(fail(ctx, PP_ERR_DRIVER, "clone copy failed"));
+    s->cur = 0;
+    *out = s;
+    return PP_OK;
+}
+
+pp_status pp_sum_len(pp_sum* s, uint64_t* len) {
+    if (!s || !len) return PP_ERR_INVALID_ARGUMENT;
+    pp_status st = refresh(s);
+    if (st != PP_OK) return st;
+    *len = s->len;
+    return PP_OK;
+}
+
+pp_status pp_sum_bucket_bits(pp_sum* s, int32_t* reference_bits, int32_t* device_bits) {
+    if (!s) return PP_ERR_INVALID_ARGUMENT;
+    if (reference_bits) *reference_bits = s->reference_bits;
+    if (device_bits) *device_bits = s->device_bits;
+    return PP_OK;
+}
+
+pp_status pp_sum_download(pp_sum* s, uint64_t* x, uint64_t* z, void* coeff) {
+    if (!s) return PP_ERR_INVALID_ARGUMENT;
+    pp_context* ctx = s->ctx;
+    {
+        pp_status rs = refresh(s);
+        if (rs != PP_OK) return rs;
+    }
+    const uint64_t n = s->len;
+    if (n == 0) return PP_OK;
+    if (!x || !z || !coeff) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "null output arrays");
+    const uint32_t W = s->W;
+    const size_t cb = s->coeff_bytes();
+    const uint32_t B = 1u << s->device_bits;
+    uint64_t *d_dense_start = nullptr, *dx = nullptr, *dz = nullptr;
+    void* dc = nullptr;
+    uint32_t* db = nullptr;
+    pp_status st = PP_OK;
+    auto cleanup = [&] {
+        if (d_dense_start) (void)hipFree(d_dense_start);
+        if (dx) (void)hipFree(dx);
+        if (dz) (void)hipFree(dz);
+        if (dc) (void)hipFree(dc);
+        if (db) (void)hipFree(db);
+    };
+    if ((st = dev_alloc(ctx, (void**)&d_dense_start, (size_t)B * 8)) != PP_OK ||
+        (st = dev_alloc(ctx, (void**)&dx, n * W * 8)) != PP_OK || (st = dev_alloc(ctx, (void**)&dz, n * W * 8)) != PP_OK ||
+        (st = dev_alloc(ctx, &dc, n * cb)) != PP_OK || (st = dev_alloc(ctx, (void**)&db, n * 4)) != PP_OK) {
+        cleanup();
+        return st;
+    }
+    hipError_t e = launch_bucket_offsets(ctx->stream, s->bk[s->cur].len, B, 0, 0, 0, 0, d_dense_start, s->d_stats + 3);
+    if (e == hipSuccess)
+        e = dispatch(s, [&](auto Wc, auto Tv) {
+            constexpr int WW = decltype(Wc)::value;
+            using T = decltype(Tv);
+            ExportArgs<WW, T> a{};
+            a.x = s->col[s->cur].x;
+            a.z = s->col[s->cur].z;
+            a.c = (const T*)s->col[s->cur].c;
+            a.cap = s->col[s->cur].cap;
+            a.start = s->bk[s->cur].start;
+            a.len = s->bk[s->cur].len;
+            a.dense_start = d_dense_start;
+            a.rows = s->d_rows;
+            a.ref_bits = s->reference_bits;
+            a.out_x = dx;
+            a.out_z = dz;
+            a.out_c = (T*)dc;
+            a.out_bucket = db;
+            return launch_export<WW, T>(ctx->stream, a, B);
+        });
+    std::vector<uint64_t> hx(n * W), hz(n * W);
+    std::vector<unsigned char> hc(n * cb);
+    std::vector<uint32_t> hb(n);
+    if (e == hipSuccess) e = hipMemcpyAsync(hx.data(), dx, n * W * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hz.data(), dz, n * W * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hc.data(), dc, n * cb, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hb.data(), db, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(ctx, PP_ERR_DRIVER, std::string("download: ") + hipGetErrorString(e));
+    // canonical order: the reference's bucket ascending (counting sort), then lexicographic key per bucket
+    const uint32_t RB = 1u << s->reference_bits;
+    std::vector<uint64_t> rstart(RB + 1, 0);
+    for (uint64_t i = 0; i < n; ++i) rstart[hb[i] + 1]++;
+    for (uint32_t b = 0; b < RB; ++b) rstart[b + 1] += rstart[b];
+    std::vector<uint64_t> order(n), cursor(rstart.begin(), rstart.end() - 1);
+    for (uint64_t i = 0; i < n; ++i) order[cursor[hb[i]]++] = i;
+    auto less = [&](uint64_t a, uint64_t b) {
+        for (uint32_t w = 0; w < W; ++w)
+            if (hx[a * W + w] != hx[b * W + w]) return hx[a * W + w] < hx[b * W + w];
+        for (uint32_t w = 0; w < W; ++w)
+            if (hz[a * W + w] != hz[b * W + w]) return hz[a * W + w] < hz[b * W + w];
+        return false;
+    };
+    for (uint32_t b = 0; b < RB; ++b) std::sort(order.begin() + rstart[b], order.begin() + rstart[b + 1], less);
+    for (uint64_t t = 0; t < n; ++t) {
+        const uint64_t i = order[t];
+        for (uint32_t w = 0; w < W; ++w) {
+            x[t * W + w] = hx[i * W + w];
+            z[t * W + w] = hz[i * W + w];
+        }
+        std::memcpy((unsigned char*)coeff + t * cb, &hc[i * cb], cb);
+    }
+    return PP_OK;
+}
+
+pp_status pp_apply_rotation(pp_sum* s, const pp_channel* ch, int adjoint, const pp_policy* policy) {
+    if (!s || !ch) return PP_ERR_INVALID_ARGUMENT;
+    if (!valid_policy(policy)) return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
+    pp_status st = refresh(s);
+    if (st != PP_OK) return st;
+    const Prepared p = prepare(s, *ch, adjoint != 0);
+    return apply_one(s, *ch, p, lower_policy(policy));
+}
+
+pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* ch, int adjoint, const pp_policy* policy) {
+    if (!src || !dst || !ch || src == dst) return PP_ERR_INVALID_ARGUMENT;
+    pp_context* ctx = src->ctx;
+    if (dst->ctx != ctx || dst->W != src->W || dst->kind != src->kind || dst->num_qubits != src->num_qubits ||
+        dst->seed != src->seed)
+        return fail(ctx, PP_ERR_INVALID_ARGUMENT, "pp_apply_rotation_to: the sums differ in shape or context");
+    if (!valid_policy(policy)) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
+    pp_status st;
+    if ((st = refresh(src)) != PP_OK) return st;
+    if ((st = settle_device_bits(src)) != PP_OK) return st;
+    const Prepared p = prepare(src, *ch, adjoint != 0);
+    if (p.key_preserving)
+        return fail(ctx, PP_ERR_UNSUPPORTED, "pp_apply_rotation_to: a key-preserving channel runs in place");
+    dst->stale = false;
+    dst->device_bits = src->device_bits;
+    dst->reference_bits = src->reference_bits;
+    if ((st = launch_join(src, dst, dst->cur, *ch, p, lower_policy(policy))) != PP_OK) return st;
+    dst->stale = true;
+    return PP_OK;
+}
+
+pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channels, pp_direction direction,
+                       const pp_policy* policy, const pp_options* options, uint64_t* layer_len) {
+    if (!s || (n_channels && !channels)) return PP_ERR_INVALID_ARGUMENT;
+    {
+        pp_status rs = refresh(s);
+        if (rs != PP_OK) return rs;
+    }
+    if (!valid_policy(policy)) return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
+    const size_t target = options ? options->target_bucket_len : 1024;
+    const size_t min_buckets = options ? options->min_buckets : 128;
+    if (target < 1 || min_buckets < 1) return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "bucket options");
+    for (uint64_t i = 0; i < n_channels; ++i)
+        for (uint32_t w = 0; w < 4; ++w) {
+            const uint64_t live = w < s->W ? live_mask(s->num_qubits, w) : 0;
+            if ((channels[i].gx[w] | channels[i].gz[w]) & ~live)
+                return fail(s->ctx, PP_ERR_INVALID_ARGUMENT,
+                            "channel " + std::to_string(i) + " acts outside the " + std::to_string(s->num_qubits) +
+                                "-qubit register");
+        }
+    const bool adjoint = direction == PP_HEISENBERG;
+    const KeepProgram keep = lower_policy(policy);
+    const bool finalizes = policy && policy->has_topn;
+    const uint64_t limit = finalizes ? policy->topn : UINT64_MAX;  // run_length_limit, builtin.rs:147-149
+    auto channel_at = [&](uint64_t i) -> const pp_channel& {
+        return channels[adjoint ? n_channels - 1 - i : i];
+    };
+    pp_status st;
+    uint64_t i = 0;
+    while (i < n_channels) {
+        // refine_unprepared (engine/backend.rs:155-165): the reference's count moves before the layer
+        int want = rebucketed_bits(s->len, s->reference_bits, target, min_buckets);
+        s->reference_bits = std::min(want, s->max_bits);
+        const Prepared p = prepare(s, channel_at(i), adjoint);
+        // Runs::take (engine/driver.rs:202-258): layers joined into a run share this bucket count
+        uint64_t end = i + 1;
+        bool run = false;
+        if (limit > 0) {
+            if (s->len <= limit && is_permutation_step(p)) {
+                uint64_t j = i + 1;
+                while (j < n_channels && is_permutation_step(prepare(s, channel_at(j), adjoint))) ++j;
+                if (j > i + 1) {
+                    end = j;
+                    run = true;
+                }
+            }
+            if (!run) {
+                auto mul = [](uint64_t a, uint64_t b) {
+                    return (a != 0 && b > UINT64_MAX / a) ? UINT64_MAX : a * b;
+                };
+                uint64_t worst = mul(s->len, fanout(p));
+                if (worst <= limit && is_rotation_table(p)) {
+                    std::vector<uint32_t> deltas{0, bucket_of_host(s, channel_at(i).gx, channel_at(i).gz,
+                                                                   s->reference_bits)};
+                    uint64_t j = i + 1;
+                    while (j < n_channels && j < i + kRunLayers) {
+                        const Prepared q = prepare(s, channel_at(j), adjoint);
+                        const uint64_t grown = mul(worst, fanout(q));
+                        if (grown > limit || !is_rotation_table(q)) break;
+                        auto next = deltas;
+                        next.push_back(0);
+                        next.push_back(bucket_of_host(s, channel_at(j).gx, channel_at(j).gz, s->reference_bits));
+                        if (span_dimension(next) > kRunDimension) break;
+                        deltas = next;
+                        worst = grown;
+                        ++j;
+                    }
+                    if (j > i + 1) end = j;
+                }
+            }
+        }
+        for (uint64_t j = i; j < end; ++j) {
+            const Prepared q = j == i ? p : prepare(s, channel_at(j), adjoint);
+            if ((st = apply_one(s, channel_at(j), q, keep)) != PP_OK) return st;
+            if (layer_len) layer_len[j] = s->len;
+        }
+        if (finalizes) {
+            if ((st = pp_truncate_topn(s, policy->topn)) != PP_OK) return st;
+            if (layer_len) layer_len[end - 1] = s->len;
+        }
+        i = end;
+    }
+    return PP_OK;
+}
+
+pp_status pp_truncate_topn(pp_sum* s, uint64_t n) {
+    if (!s) return PP_ERR_INVALID_ARGUMENT;
+    pp_status rs = refresh(s);
+    if (rs != PP_OK) return rs;
+    if (s->len <= n) return PP_OK;
+    return topn_device(s, n);
+}
+
+pp_status pp_expectation_product(pp_sum* s, const uint64_t* axis_x, const uint64_t* axis_z, const uint64_t* negative,
+                                 double* out) {
+    if (!s || !axis_x || !axis_z || !negative || !out) return PP_ERR_INVALID_ARGUMENT;
+    pp_context* ctx = s->ctx;
+    {
+        pp_status rs = refresh(s);
+        if (rs != PP_OK) return rs;
+    }
+    const uint32_t B = 1u << s->device_bits;
+    const uint32_t per = 64;
+    const uint32_t blocks = (B + per - 1) / per;
+    double* d_partial = nullptr;
+    pp_status st;
+    if ((st = dev_alloc(ctx, (void**)&d_partial, blocks * 8)) != PP_OK) return st;
+    hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
+        constexpr int W = decltype(Wc)::value;
+        using T = decltype(Tv);
+        ExpectationArgs<W, T> a{};
+        a.x = s->col[s->cur].x;
+        a.z = s->col[s->cur].z;
+        a.c = (const T*)s->col[s->cur].c;
+        a.cap = s->col[s->cur].cap;
+        a.start = s->bk[s->cur].start;
+        a.len = s->bk[s->cur].len;
+        a.num_buckets = B;
+        a.buckets_per_block = per;
+        for (int w = 0; w < W; ++w) {
+            a.axis_x[w] = axis_x[w];
+            a.axis_z[w] = axis_z[w];
+            a.negative[w] = negative[w];
+        }
+        a.partial = d_partial;
+        return launch_expectation<W, T>(ctx->stream, a, blocks);
+    });
+    std::vector<double> partial(blocks);
+    if (e == hipSuccess) e = hipMemcpyAsync(partial.data(), d_partial, blocks * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_partial);
+    if (e != hipSuccess) return fail(ctx, PP_ERR_DRIVER, std::string("expectation: ") + hipGetErrorString(e));
+    double total = 0.0;
+    for (double v : partial) total += v;
+    *out = total;
+    return PP_OK;
+}
+
+pp_status pp_profile_enable(pp_context* ctx, int enable) {
+    if (!ctx) return PP_ERR_INVALID_ARGUMENT;
+    PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& e : ctx->events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    ctx->events.clear();
+    ctx->prof_rows_in = 0;
+    if (!ctx->d_prof) {
+        pp_status st = dev_alloc(ctx, (void**)&ctx->d_prof, 2 * sizeof(unsigned long long));
+        if (st != PP_OK) return st;
+    }
+    PP_HIP(ctx, hipMemsetAsync(ctx->d_prof, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->profiling = enable != 0;
+    return PP_OK;
+}
+
+pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms, uint64_t* rows_in,
+                          uint64_t* rows_out) {
+    if (!ctx) return PP_ERR_INVALID_ARGUMENT;
+    PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double ms = 0.0;
+    for (auto& e : ctx->events) {
+        float t = 0.f;
+        PP_HIP(ctx, hipEventElapsedTime(&t, e.first, e.second));
+        ms += t;
+    }
+    if (launches) *launches = ctx->events.size();
+    if (total_ms) *total_ms = ms;
+    if (rows_in) *rows_in = ctx->prof_rows_in;
+    unsigned long long out_rows = 0;
+    if (ctx->d_prof) PP_HIP(ctx, hipMemcpy(&out_rows, ctx->d_prof, sizeof(out_rows), hipMemcpyDeviceToHost));
+    if (rows_out) *rows_out = out_rows;
+    return PP_OK;
+}
+
+}  // extern "C"
+
+// The TopN pass lives in pp_topn.hip; it needs the sum's internals.
+#include "pp_topn_impl.inc"

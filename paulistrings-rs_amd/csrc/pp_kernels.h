@@ -1,0 +1,148 @@
+// pp_kernels.h -- argument blocks and launchers of the rotation-layer kernels (pp_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pp_device.h"
+
+namespace pp {
+
+// Columns are word-major: x[w * cap + i], z[w * cap + i], c[i].  Buckets are CSR segments start[b], len[b].
+
+template <int W, typename T>
+struct JoinArgs {
+    const uint64_t* x;
+    const uint64_t* z;
+    const T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    const uint32_t* len;
+    uint64_t* ox;
+    uint64_t* oz;
+    T* oc;
+    uint64_t ocap;
+    const uint64_t* ostart;
+    uint32_t* olen;
+    uint32_t delta;      // h(g) under the device bucket bits; 0 makes every coset a single bucket
+    uint32_t hb;         // highest set bit of delta
+    uint32_t lcap;       // LDS rows: at least the largest pair
+    uint32_t tcap_bits;  // LDS table slots: 1 << tcap_bits >= 2 * lcap
+    uint64_t gx[W];
+    uint64_t gz[W];
+    T cos_t;
+    T sin_t;
+    int32_t clifford;
+    KeepProgram keep;
+    unsigned long long* stats;  // [0] += output rows, [1] = max output bucket, [2] |= LDS overflow
+    unsigned long long* prof;   // null, or [0] += output rows across launches (measurement)
+};
+
+template <int W, typename T>
+struct RescaleArgs {
+    uint64_t* x;
+    uint64_t* z;
+    T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    uint32_t* len;
+    uint64_t gx[W];
+    uint64_t gz[W];
+    T a0;
+    KeepProgram keep;
+    unsigned long long* stats;
+};
+
+template <int W, typename T>
+struct RefineArgs {
+    const uint64_t* x;
+    const uint64_t* z;
+    const T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    const uint32_t* len;
+    uint64_t* ox;
+    uint64_t* oz;
+    T* oc;
+    uint64_t ocap;
+    uint64_t* ostart;  // 2 * num_buckets entries
+    uint32_t* olen;
+    uint32_t num_buckets;  // before the split
+    int32_t row;           // the hash row that splits
+    const uint64_t* rows;  // [row][x|z][W]
+    unsigned long long* stats;
+};
+
+template <int W, typename T>
+struct CoarsenArgs {
+    const uint64_t* x;
+    const uint64_t* z;
+    const T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    const uint32_t* len;
+    uint64_t* ox;
+    uint64_t* oz;
+    T* oc;
+    uint64_t ocap;
+    const uint64_t* ostart;  // coarse_buckets entries, from k_bucket_offsets mode 2
+    uint32_t* olen;
+    uint32_t num_buckets;
+    uint32_t coarse_buckets;
+    unsigned long long* stats;
+};
+
+template <int W, typename T>
+struct ExpectationArgs {
+    const uint64_t* x;
+    const uint64_t* z;
+    const T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    const uint32_t* len;
+    uint32_t num_buckets;
+    uint32_t buckets_per_block;
+    uint64_t axis_x[W];
+    uint64_t axis_z[W];
+    uint64_t negative[W];
+    double* partial;
+};
+
+template <int W, typename T>
+struct ExportArgs {
+    const uint64_t* x;
+    const uint64_t* z;
+    const T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    const uint32_t* len;
+    const uint64_t* dense_start;
+    const uint64_t* rows;
+    int32_t ref_bits;
+    uint64_t* out_x;  // term-major [n][W]
+    uint64_t* out_z;
+    T* out_c;
+    uint32_t* out_bucket;
+};
+
+hipError_t launch_bucket_offsets(hipStream_t st, const uint32_t* len, uint32_t num_buckets, uint32_t mode,
+                                 uint32_t delta, uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start,
+                                 unsigned long long* total);
+
+template <int W, typename T>
+size_t join_lds_bytes(uint32_t lcap, uint32_t tcap_bits);
+template <int W, typename T>
+hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t num_cosets);
+template <int W, typename T>
+hipError_t launch_rescale(hipStream_t st, const RescaleArgs<W, T>& a, uint32_t num_buckets);
+template <int W, typename T>
+hipError_t launch_refine(hipStream_t st, const RefineArgs<W, T>& a);
+template <int W, typename T>
+hipError_t launch_coarsen(hipStream_t st, const CoarsenArgs<W, T>& a);
+template <int W, typename T>
+hipError_t launch_expectation(hipStream_t st, const ExpectationArgs<W, T>& a, uint32_t blocks);
+template <int W, typename T>
+hipError_t launch_export(hipStream_t st, const ExportArgs<W, T>& a, uint32_t num_buckets);
+
+}  // namespace pp

@@ -1,0 +1,84 @@
+// pp_topn.h -- TopN as a radix select on the bits of |c|^2 (truncation/builtin.rs:73-129; the reference's device
+// form is engine/gpu/finalize.rs:238-409 with kern/truncate.cu:31-141).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pp_device.h"
+
+namespace pp {
+
+// Device-resident state of one selection, advanced by k_topn_select after every histogram pass.
+struct TopnState {
+    unsigned long long prefix;   // the digits of the n-th largest |c|^2 found so far
+    unsigned long long mask;     // which bits of prefix are decided
+    unsigned long long k;        // rank still sought among the terms matching prefix (1 = the largest)
+    unsigned long long above;    // terms strictly above the boundary found so far
+    unsigned long long finite;   // terms whose |c|^2 is not NaN
+    unsigned long long ties;     // terms at the final boundary
+    unsigned long long keep_all_finite;  // at most n finite terms: keep every one of them
+    unsigned long long gathered;  // tie terms written by k_topn_gather_ties
+};
+
+constexpr int kTopnDigitBits = 11;
+constexpr int kTopnBins = 1 << kTopnDigitBits;
+constexpr int kTopnPasses = 6;  // digits at shifts 53, 42, 31, 20, 9 (11 bits) and 0 (9 bits)
+
+__host__ __device__ inline int topn_shift(int pass) { return pass < 5 ? 53 - 11 * pass : 0; }
+__host__ __device__ inline int topn_bits(int pass) { return pass < 5 ? 11 : 9; }
+
+template <typename T>
+struct TopnHistArgs {
+    const T* c;
+    const uint64_t* start;
+    const uint32_t* len;
+    uint32_t num_buckets;
+    uint32_t buckets_per_block;
+    int32_t pass;
+    const TopnState* state;
+    unsigned long long* hist;  // kTopnBins
+};
+
+template <int W, typename T>
+struct TopnTieArgs {
+    const uint64_t* x;
+    const uint64_t* z;
+    const T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    const uint32_t* len;
+    const uint64_t* rows;
+    int32_t ref_bits;
+    TopnState* state;
+    uint64_t* tie_x;  // term-major [ties][W]
+    uint64_t* tie_z;
+    uint32_t* tie_bucket;
+    uint64_t* tie_pos;
+    uint64_t tie_cap;
+};
+
+template <int W, typename T>
+struct TopnRetainArgs {
+    uint64_t* x;
+    uint64_t* z;
+    T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    uint32_t* len;
+    const TopnState* state;
+    const uint8_t* tie_keep;  // per column position; null keeps every tie
+    unsigned long long* stats;
+};
+
+template <typename T>
+hipError_t launch_topn_hist(hipStream_t st, const TopnHistArgs<T>& a, uint32_t blocks);
+hipError_t launch_topn_select(hipStream_t st, TopnState* state, unsigned long long* hist, int pass, uint64_t n);
+hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n);
+template <int W, typename T>
+hipError_t launch_topn_gather_ties(hipStream_t st, const TopnTieArgs<W, T>& a, uint32_t num_buckets);
+template <int W, typename T>
+hipError_t launch_topn_retain(hipStream_t st, const TopnRetainArgs<W, T>& a, uint32_t num_buckets);
+hipError_t launch_mark_positions(hipStream_t st, const uint64_t* pos, uint64_t n, uint8_t* flags);
+
+}  // namespace pp

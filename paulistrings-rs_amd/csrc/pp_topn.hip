@@ -1,0 +1,202 @@
+// pp_topn.hip -- TopN kernels: a six-pass radix select on the IEEE bits of |c|^2 (non-negative doubles order as
+// their bits), the gather of the terms tied at the boundary, and the in-place retain.
+//
+// Reference semantics (truncation/builtin.rs:73-129, :152-239): with t the n-th largest |c|^2, keep every term above
+// t and the first n - above of the terms at t in canonical order (the reference's bucket ascending, then the
+// lexicographic key); a NaN is never kept, and with at most n other terms every non-NaN term is kept.
+#include "pp_topn.h"
+
+namespace pp {
+
+__device__ __forceinline__ bool is_nan_bits(uint64_t u) { return (u & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull; }
+
+template <typename T>
+__device__ __forceinline__ uint64_t magnitude_bits(T c) {
+    const double v = (double)c;  // norm_sqr widens to f64 first (coefficient.rs:121-125)
+    return (uint64_t)__double_as_longlong(v * v);
+}
+
+__global__ void k_topn_init(TopnState* s, uint64_t n) {
+    s->prefix = 0;
+    s->mask = 0;
+    s->k = n;
+    s->above = 0;
+    s->finite = 0;
+    s->ties = 0;
+    s->keep_all_finite = 0;
+    s->gathered = 0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
+    __shared__ uint32_t h[kTopnBins];
+    for (int i = threadIdx.x; i < kTopnBins; i += kBlock) h[i] = 0;
+    __syncthreads();
+    const TopnState st = *a.state;
+    if (!st.keep_all_finite) {
+        const int shift = topn_shift(a.pass);
+        const uint64_t dmask = (1ull << topn_bits(a.pass)) - 1ull;
+        const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
+        const uint32_t b_hi = min(a.num_buckets, b_lo + a.buckets_per_block);
+        for (uint32_t b = b_lo; b < b_hi; ++b) {
+            const uint32_t n = a.len[b];
+            const uint64_t s = a.start[b];
+            for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
+                const uint64_t u = magnitude_bits<T>(a.c[s + r]);
+                if (!is_nan_bits(u) && (u & st.mask) == st.prefix) atomicAdd(&h[(u >> shift) & dmask], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTopnBins; i += kBlock)
+        if (h[i]) atomicAdd(&a.hist[i], (unsigned long long)h[i]);
+}
+
+// One workgroup: pick the digit holding the k-th largest, then clear the histogram for the next pass.
+__global__ void __launch_bounds__(kBlock) k_topn_select(TopnState* s, unsigned long long* hist, int pass, uint64_t n) {
+    if (threadIdx.x == 0 && !s->keep_all_finite) {
+        const int bins = 1 << topn_bits(pass);
+        if (pass == 0) {
+            unsigned long long total = 0;
+            for (int d = 0; d < bins; ++d) total += hist[d];
+            s->finite = total;
+            if (total <= n) s->keep_all_finite = 1;
+        }
+        if (!s->keep_all_finite) {
+            unsigned long long acc = 0;
+            int chosen = 0;
+            for (int d = bins - 1; d >= 0; --d) {
+                const unsigned long long c = hist[d];
+                if (acc + c >= s->k) {
+                    chosen = d;
+                    break;
+                }
+                acc += c;
+            }
+            const int shift = topn_shift(pass);
+            s->above += acc;
+            s->k -= acc;
+            s->prefix |= (unsigned long long)chosen << shift;
+            s->mask |= ((1ull << topn_bits(pass)) - 1ull) << shift;
+            s->ties = hist[chosen];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTopnBins; i += kBlock) hist[i] = 0;
+}
+
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_topn_gather_ties(const TopnTieArgs<W, T> a) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = a.len[b];
+    const uint64_t s = a.start[b];
+    const uint64_t t = a.state->prefix;
+    for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
+        const uint64_t u = magnitude_bits<T>(a.c[s + r]);
+        if (u != t) continue;
+        const uint64_t slot = atomicAdd(&a.state->gathered, 1ull);
+        if (slot >= a.tie_cap) continue;
+        const Key<W> k = load_key<W>(a.x, a.z, a.cap, s + r);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            a.tie_x[slot * W + w] = k.x[w];
+            a.tie_z[slot * W + w] = k.z[w];
+        }
+        a.tie_bucket[slot] = bucket_of<W>(k, a.rows, a.ref_bits);
+        a.tie_pos[slot] = s + r;
+    }
+}
+
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_topn_retain(const TopnRetainArgs<W, T> a) {
+    __shared__ uint32_t scan_scratch[kWaves];
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = a.len[b];
+    const uint64_t s = a.start[b];
+    const TopnState st = *a.state;
+    uint32_t cur = 0;
+    for (uint32_t base = 0; base < n; base += kBlock) {
+        const uint32_t r = base + threadIdx.x;
+        Key<W> k;
+        T c = T(0);
+        bool keep = false;
+        if (r < n) {
+            k = load_key<W>(a.x, a.z, a.cap, s + r);
+            c = a.c[s + r];
+            const uint64_t u = magnitude_bits<T>(c);
+            if (st.keep_all_finite) keep = !is_nan_bits(u);
+            else if (is_nan_bits(u)) keep = false;
+            else if (u > st.prefix) keep = true;
+            else if (u == st.prefix) keep = a.tie_keep == nullptr || a.tie_keep[s + r] != 0;
+        }
+        bool f[1] = {keep};
+        uint32_t off[1], tot[1];
+        block_flag_scan<1>(f, off, tot, scan_scratch);  // every read of the chunk precedes its writes
+        if (keep && cur + off[0] != r) {
+            const uint64_t pos = s + cur + off[0];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                a.x[w * a.cap + pos] = k.x[w];
+                a.z[w * a.cap + pos] = k.z[w];
+            }
+            a.c[pos] = c;
+        }
+        cur += tot[0];
+    }
+    if (threadIdx.x == 0) {
+        a.len[b] = cur;
+        if (cur) atomicAdd(&a.stats[0], (unsigned long long)cur);
+        atomicMax(&a.stats[1], (unsigned long long)cur);
+    }
+}
+
+__global__ void k_mark_positions(const uint64_t* pos, uint64_t n, uint8_t* flags) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[pos[i]] = 1;
+}
+
+template <typename T>
+hipError_t launch_topn_hist(hipStream_t st, const TopnHistArgs<T>& a, uint32_t blocks) {
+    hipLaunchKernelGGL((k_topn_hist<T>), dim3(blocks), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_topn_select(hipStream_t st, TopnState* state, unsigned long long* hist, int pass, uint64_t n) {
+    hipLaunchKernelGGL(k_topn_select, dim3(1), dim3(kBlock), 0, st, state, hist, pass, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n) {
+    hipLaunchKernelGGL(k_topn_init, dim3(1), dim3(1), 0, st, state, n);
+    return hipGetLastError();
+}
+
+template <int W, typename T>
+hipError_t launch_topn_gather_ties(hipStream_t st, const TopnTieArgs<W, T>& a, uint32_t num_buckets) {
+    hipLaunchKernelGGL((k_topn_gather_ties<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int W, typename T>
+hipError_t launch_topn_retain(hipStream_t st, const TopnRetainArgs<W, T>& a, uint32_t num_buckets) {
+    hipLaunchKernelGGL((k_topn_retain<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_positions(hipStream_t st, const uint64_t* pos, uint64_t n, uint8_t* flags) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mark_positions, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, pos, n, flags);
+    return hipGetLastError();
+}
+
+template hipError_t launch_topn_hist<double>(hipStream_t, const TopnHistArgs<double>&, uint32_t);
+template hipError_t launch_topn_hist<float>(hipStream_t, const TopnHistArgs<float>&, uint32_t);
+#define PP_TOPN_INSTANTIATE(W, T)                                                                       \
+    template hipError_t launch_topn_gather_ties<W, T>(hipStream_t, const TopnTieArgs<W, T>&, uint32_t); \
+    template hipError_t launch_topn_retain<W, T>(hipStream_t, const TopnRetainArgs<W, T>&, uint32_t);
+PP_TOPN_INSTANTIATE(1, double)
+PP_TOPN_INSTANTIATE(1, float)
+PP_TOPN_INSTANTIATE(2, double)
+PP_TOPN_INSTANTIATE(2, float)
+
+}  // namespace pp

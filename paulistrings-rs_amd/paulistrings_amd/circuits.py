@@ -88,6 +88,11 @@ class Circuit:
     def rzz(self, theta: float, a: int, b: int) -> "Circuit":
         return self.pauli_rotation("ZZ", [a, b], theta)
 
+    def append(self, gate) -> "Circuit":
+        """Append a gate from :mod:`paulistrings_amd.gates`."""
+        gate.apply_to(self)
+        return self
+
     def extend(self, other: "Circuit") -> "Circuit":
         if other.num_qubits != self.num_qubits:
             raise ValueError("circuits act on different registers")

@@ -1,0 +1,95 @@
+"""The C ABI library builds, loads and exports every entry point include/pp_hip.h declares (no GPU needed),
+and the host halves of the product (hash rows, policy lowering, circuits) agree with the oracle."""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import pp_oracle as po
+from paulistrings_amd import Circuit, PauliSum, _native, circuits, truncation
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "pp_hip.h").read_text()
+    return sorted(set(re.findall(r"\b(pp_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    syms = declared_symbols()
+    assert "pp_propagate" in syms and "pp_sum_upload" in syms
+    L = _native.lib()
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_context_without_a_device_reports_no_device():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a device is visible")
+    h = ctypes.c_void_p()
+    assert _native.lib().pp_context_create(0, ctypes.byref(h)) == 1  # PP_ERR_NO_DEVICE
+
+
+@pytest.mark.parametrize("n,width", [(1, 1), (5, 1), (36, 1), (64, 1), (100, 2), (127, 2)])
+def test_product_hash_rows_equal_the_oracle(n, width):
+    rx = np.zeros(31 * width, np.uint64)
+    rz = np.zeros(31 * width, np.uint64)
+    cnt = _native.lib().pp_hash_rows(n, 0x9E3779B97F4A7C15, width, rx.ctypes.data, rz.ctypes.data)
+    ox, oz = po.hash_rows(n, 0x9E3779B97F4A7C15, width)
+    assert cnt == len(ox)
+    assert np.array_equal(rx[: cnt * width].reshape(cnt, width), ox)
+    assert np.array_equal(rz[: cnt * width].reshape(cnt, width), oz)
+
+
+def test_policy_lowering_is_postfix():
+    p = (truncation.coeff(1e-4) & truncation.weight(8)).lower()
+    assert [(p.nodes[i].op, p.nodes[i].k) for i in range(p.n_nodes)] == [(1, 0), (2, 8), (3, 0)]
+    assert p.has_topn == 0
+    q = (truncation.coeff(1e-5) & truncation.topn(1 << 24)).lower()
+    assert q.n_nodes == 1 and q.has_topn == 1 and q.topn == 1 << 24
+    with pytest.raises(ValueError):
+        truncation.coeff(1e-3) | truncation.topn(5)
+    r = (truncation.coeff(1e-2) | truncation.weight(3)).lower()
+    assert [r.nodes[i].op for i in range(r.n_nodes)] == [1, 2, 4]
+
+
+def test_pauli_sum_construction_sums_duplicates_and_drops_zeros():
+    s = PauliSum.from_strings({"XI": 1.0, "ZZ": 0.5}, num_qubits=2)
+    x = np.array([[1], [1], [0]], np.uint64)
+    z = np.array([[0], [0], [3]], np.uint64)
+    t = PauliSum(2, x, z, np.array([0.25, -0.25, 2.0]))
+    assert len(s) == 2 and t.to_dict() == {((0,), (3,)): 2.0}
+
+
+def test_repeated_keys_sum_in_input_order():
+    """Long runs of one key sum one addition at a time, as the reference's hashmap accumulator and the oracle do."""
+    rng = np.random.default_rng(1012)
+    n = 3000
+    x = (rng.integers(0, 8, size=(n, 1))).astype(np.uint64)
+    z = (rng.integers(0, 4, size=(n, 1))).astype(np.uint64)
+    c = rng.standard_normal(n)
+    host = PauliSum(12, x, z, c)
+    ox, oz, oc, _ = po.OracleSum(12, x, z, c).export()
+    order = np.lexsort((oz[:, 0], ox[:, 0]))
+    assert np.array_equal(host.x_array(), ox[order]) and np.array_equal(host.coefficients_array(), oc[order])
+
+
+def test_circuit_encoding():
+    c = Circuit(70).pauli_rotation("XZ", [69, 3], 0.5).rx(0.1, 64)
+    ch = c.channels()
+    assert int(ch[0]["gx"][1]) == 1 << 5 and int(ch[0]["gz"][0]) == 1 << 3 and ch[0]["theta"] == 0.5
+    assert int(ch[1]["gx"][1]) == 1
+    assert len(circuits.heavy_hex_kicked_ising(1, 0.1)) == 127 + 144
+    assert len(circuits.tfim_square_step(6, 6, 0.1, 1.0, 3.0)) == 108
+    with pytest.raises(ValueError):
+        Circuit(4).rx(0.1, 4)
+    assert math.isclose(circuits.tfim_square_step(2, 2, 0.05, 1.0, 1.0).channels()[-1]["theta"], 0.1)

@@ -1,0 +1,239 @@
+"""Parity of the HIP path (through the C ABI) with the oracle: keys bit-exact, coefficients bit-exact (every
+output has at most two contributions, so no summation order is involved), canonical order identical, and the
+reference's golden kicked-Ising records reproduced.  Each case prints its name before it touches the GPU."""
+
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import golden_data
+import pp_oracle as po
+from paulistrings_amd import Circuit, PauliSum, circuits, truncation
+
+pytestmark = pytest.mark.gpu
+
+F64 = np.float64
+F32 = np.float32
+
+
+def announce(msg: str) -> None:
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def oracle_policy(eps=None, max_weight=None, topn=None):
+    return po.coeff_policy(eps=eps, max_weight=max_weight, topn=topn)
+
+
+def product_policy(eps=None, max_weight=None, topn=None):
+    parts = []
+    if eps is not None:
+        parts.append(truncation.coeff(eps))
+    if max_weight is not None:
+        parts.append(truncation.weight(max_weight))
+    if topn is not None:
+        parts.append(truncation.topn(topn))
+    if not parts:
+        return None
+    p = parts[0]
+    for q in parts[1:]:
+        p = p & q
+    return p
+
+
+def run_both(num_qubits, x, z, c, circ: Circuit, direction="forward", eps=None, max_weight=None, topn=None):
+    s = po.OracleSum(num_qubits, x, z, c)
+    olens = s.propagate(circ.channels(), direction, oracle_policy(eps, max_weight, topn))
+    ox, oz, oc, _ = s.export()
+    host = PauliSum(num_qubits, x, z, c, dtype=c.dtype)
+    stats = host.propagate_with_stats(circ, product_policy(eps, max_weight, topn), direction)
+    return (ox, oz, oc, olens), (host.x_array(), host.z_array(), host.coefficients_array(), stats)
+
+
+def assert_same(oracle, product):
+    ox, oz, oc, olens = oracle
+    px, pz, pc, stats = product
+    assert len(pc) == len(oc), (len(pc), len(oc))
+    assert np.array_equal(px, ox) and np.array_equal(pz, oz), "keys or canonical order differ"
+    assert np.array_equal(pc, oc), f"coefficients differ: max |d| = {np.max(np.abs(pc.astype(float) - oc)) if len(oc) else 0}"
+    assert [int(v) for v in olens] == stats.terms_out
+
+
+def random_sum(rng, n, num_qubits, density, dtype):
+    W = circuits.width_for(num_qubits)
+    x = np.zeros((n, W), np.uint64)
+    z = np.zeros((n, W), np.uint64)
+    for w in range(W):
+        live = min(64, num_qubits - 64 * w)
+        if live <= 0:
+            continue
+        weights = np.uint64(1) << np.arange(live, dtype=np.uint64)
+        x[:, w] = ((rng.random((n, live)) < density) * weights).sum(axis=1, dtype=np.uint64)
+        z[:, w] = ((rng.random((n, live)) < density) * weights).sum(axis=1, dtype=np.uint64)
+    return x, z, rng.standard_normal(n).astype(dtype)
+
+
+def random_circuit(rng, num_qubits, layers):
+    c = Circuit(num_qubits)
+    for _ in range(layers):
+        k = int(rng.integers(1, 3))
+        qs = [int(q) for q in rng.choice(num_qubits, size=k, replace=False)]
+        ps = "".join(rng.choice(list("XYZ"), size=k))
+        theta = float(rng.choice([rng.uniform(-math.pi, math.pi), -math.pi / 2, math.pi / 2, math.pi]))
+        c.pauli_rotation(ps, qs, theta)
+    return c
+
+
+def test_hand_computed_single_gates():
+    announce("hand-computed single gates")
+    t = 0.6
+    cases = [
+        ({"X": 1.0}, "Z", [0], t, "forward", {"X": math.cos(t), "Y": math.sin(t)}),
+        ({"Z": 1.0}, "X", [0], t, "forward", {"Z": math.cos(t), "Y": -math.sin(t)}),
+        ({"XI": 1.0}, "ZZ", [0, 1], t, "forward", {"XI": math.cos(t), "YZ": math.sin(t)}),
+        ({"XI": 1.0}, "ZZ", [0, 1], -math.pi / 2, "forward", {"YZ": -1.0}),
+        ({"X": 1.0}, "Z", [0], t, "heisenberg", {"X": math.cos(t), "Y": -math.sin(t)}),
+    ]
+    for terms, paulis, qubits, theta, direction, want in cases:
+        n = len(next(iter(terms)))
+        s = PauliSum.from_strings(terms, num_qubits=n)
+        s.propagate(Circuit(n).pauli_rotation(paulis, qubits, theta), direction=direction)
+        got = {}
+        for (xw, zw), v in s.to_dict().items():
+            got["".join("IXZY"[((xw[0] >> q) & 1) | (((zw[0] >> q) & 1) << 1)] for q in range(n))] = v
+        assert got == want
+
+
+def test_rotation_amplitudes_bitwise():
+    """cos and sin of 1024 random angles as the layer applies them: X_q under RZ(theta_q) becomes
+    cos X_q + sin Y_q, so the coefficients are the host libm's values, compared bitwise with the oracle's."""
+    announce("rotation amplitudes of 1024 angles")
+    rng = np.random.default_rng(17)
+    x = np.array([[1 << q] for q in range(64)], np.uint64)
+    z = np.zeros((64, 1), np.uint64)
+    c = np.ones(64)
+    for _ in range(16):
+        circ = Circuit(64)
+        for q in range(64):
+            circ.rz(float(rng.uniform(-4.0, 4.0)), q)
+        o, p = run_both(64, x, z, c, circ, "forward")
+        assert_same(o, p)
+
+
+@pytest.mark.parametrize("num_qubits", [12, 64, 65, 127])
+@pytest.mark.parametrize("dtype", [F64, F32])
+@pytest.mark.parametrize("direction", ["forward", "heisenberg"])
+def test_random_circuits_bitwise(num_qubits, dtype, direction):
+    announce(f"random circuit n={num_qubits} {np.dtype(dtype).name} {direction}")
+    rng = np.random.default_rng(1000 + num_qubits)
+    x, z, c = random_sum(rng, 3000, num_qubits, 0.15, dtype)
+    circ = random_circuit(rng, num_qubits, 40)
+    o, p = run_both(num_qubits, x, z, c, circ, direction, eps=1e-3, max_weight=num_qubits // 2)
+    assert_same(o, p)
+
+
+@pytest.mark.parametrize("dtype", [F64, F32])
+def test_untruncated_growth_bitwise(dtype):
+    announce(f"untruncated growth {np.dtype(dtype).name}")
+    rng = np.random.default_rng(7)
+    x, z, c = random_sum(rng, 200, 20, 0.2, dtype)
+    circ = random_circuit(rng, 20, 14)
+    o, p = run_both(20, x, z, c, circ, "forward")
+    assert_same(o, p)
+
+
+def test_tfim_6x6_ten_steps():
+    """The CPU-runnable configuration: 6x6 periodic TFIM, J=1, h=3, dt=0.1, 10 steps, threshold 1e-4 and
+    weight 8, average X, read out in |+>^36."""
+    announce("6x6 TFIM 10 steps")
+    step = circuits.tfim_square_step(6, 6, 0.1, 1.0, 3.0)
+    circ = circuits.repeat(step, 10)
+    n = 36
+    x = np.array([[1 << q] for q in range(n)], np.uint64)
+    z = np.zeros((n, 1), np.uint64)
+    c = np.full(n, 1.0 / n)
+    o, p = run_both(n, x, z, c, circ, "heisenberg", eps=1e-4, max_weight=8)
+    assert_same(o, p)
+    s = po.OracleSum(n, x, z, c)
+    s.propagate(circ.channels(), "heisenberg", oracle_policy(1e-4, 8))
+    host = PauliSum(n, p[0], p[1], p[2])
+    assert abs(host.expectation("x+") - s.expectation("x")) < 1e-12
+
+
+GOLDEN = [r for r in golden_data.records() if r["peak_terms"] <= 700_000]
+
+
+@pytest.mark.parametrize("rec", GOLDEN[::3], ids=lambda r: f"{r['observable']}-{r['theta_h_label']}-{r['trotter_steps']}"
+                         f"-{r['min_abs_coeff']}-{r['max_weight']}")
+def test_golden_kicked_ising(rec):
+    announce(f"golden {rec['observable']} {rec['theta_h_label']} steps={rec['trotter_steps']} "
+             f"eps={rec['min_abs_coeff']} w={rec['max_weight']}")
+    x, z, c = golden_data.observable(rec["observable"])
+    circ = circuits.heavy_hex_kicked_ising(rec["trotter_steps"], rec["theta_h"])
+    g = PauliSum(127, x, z, c).to_device(0)
+    stats = g.propagate_with_stats(circ, product_policy(rec["min_abs_coeff"], rec["max_weight"]), "heisenberg")
+    assert len(g) == rec["final_terms"]
+    assert stats.peak_terms == rec["peak_terms"]
+    assert abs(g.expectation("z+") - rec["expectation_value"]) <= 1e-12 * max(1.0, abs(rec["expectation_value"]))
+    g.free()
+
+
+@pytest.mark.parametrize("n", [1, 700, 2047, 5000])
+def test_topn_ties_in_canonical_order(n):
+    announce(f"topn ties n={n}")
+    rng = np.random.default_rng(3)
+    x, z, _ = random_sum(rng, 12000, 40, 0.3, F64)
+    c = rng.choice([0.5, -0.5, 0.25, 0.125, -0.125, 1e-3], size=12000)
+    s = po.OracleSum(40, x, z, c)
+    s.topn(n)
+    ox, oz, oc, _ = s.export()
+    g = PauliSum(40, x, z, c).to_device(0)
+    g.truncate_topn(n)
+    h = g.to_host()
+    g.free()
+    assert np.array_equal(h.x_array(), ox) and np.array_equal(h.z_array(), oz)
+    assert np.array_equal(h.coefficients_array(), oc)
+
+
+def test_topn_inside_propagation():
+    announce("topn inside a propagation")
+    rng = np.random.default_rng(9)
+    x, z, c = random_sum(rng, 64, 30, 0.1, F64)
+    circ = random_circuit(rng, 30, 30)
+    o, p = run_both(30, x, z, c, circ, "heisenberg", eps=1e-6, topn=900)
+    assert_same(o, p)
+
+
+def test_edge_cases():
+    announce("edge cases")
+    # empty sum
+    e = PauliSum(10)
+    e.propagate(Circuit(10).rx(0.3, 2))
+    assert len(e) == 0
+    # the identity generator and quarter turns
+    rng = np.random.default_rng(4)
+    x, z, c = random_sum(rng, 500, 10, 0.3, F64)
+    circ = Circuit(10).pauli_rotation("", [], 0.7).rz(math.pi, 3).rx(-math.pi / 2, 4).rzz(math.pi / 2, 0, 9)
+    o, p = run_both(10, x, z, c, circ, "forward", eps=1e-9)
+    assert_same(o, p)
+    # generators straddling the word boundary, the last live qubit of W = 2
+    x, z, c = random_sum(rng, 2000, 128, 0.05, F64)
+    circ = Circuit(128).pauli_rotation("XZ", [63, 64], 0.4).pauli_rotation("YY", [127, 0], 1.1).rx(0.2, 127)
+    o, p = run_both(128, x, z, c, circ, "forward")
+    assert_same(o, p)
+
+
+def test_saturated_layer_reduced():
+    """The saturated-layer workload at 2^16 keys: 127 qubits, bits Bernoulli(0.1), N(0,1) coefficients, the first
+    RZZ and RX of the dense layer, threshold 1e-6, f64 and f32."""
+    announce("saturated layer 2^16")
+    rng = np.random.default_rng(0xC0FFEE)
+    for dtype in (F64, F32):
+        x, z, c = random_sum(rng, 1 << 16, 127, 0.1, dtype)
+        circ = Circuit(127).rzz(float(rng.uniform(0, math.pi)), 0, 1).rx(float(rng.uniform(0, math.pi)), 0)
+        o, p = run_both(127, x, z, c, circ, "forward", eps=1e-6)
+        assert_same(o, p)

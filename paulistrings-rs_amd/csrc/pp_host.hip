@@ -185,6 +185,8 @@ struct pp_sum {
     uint64_t len = 0;
     uint32_t max_bucket = 0;
     bool stale = false;  // len and max_bucket await the last layer's counters (pp_apply_rotation_to)
+    uint64_t* d_tiles = nullptr;  // scratch of the bucket-offset scan
+    uint64_t tiles_cap = 0;
     unsigned long long* d_stats = nullptr;
     unsigned long long* h_stats = nullptr;  // pinned
     pp::TopnState* d_topn = nullptr;
@@ -292,6 +294,39 @@ pp_status clear_stats(pp_sum* s) {
     return PP_OK;
 }
 
+// Scratch for the bucket-offset scan of a layout of `out_n` buckets.
+pp_status ensure_tiles(pp_sum* s, uint32_t out_n) {
+    const uint64_t need = std::max<uint32_t>(offsets_tiles(out_n), 1);
+    if (s->tiles_cap >= need && s->d_tiles) return PP_OK;
+    if (s->d_tiles) (void)hipFree(s->d_tiles);
+    s->d_tiles = nullptr;
+    s->tiles_cap = 0;
+    pp_status st = dev_alloc(s->ctx, (void**)&s->d_tiles, need * 8);
+    if (st != PP_OK) return st;
+    s->tiles_cap = need;
+    return PP_OK;
+}
+
+pp_status offsets(pp_sum* s, const uint32_t* len, uint32_t num_buckets, uint32_t mode, uint32_t delta,
+                  uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start) {
+    pp_status st;
+    if ((st = ensure_tiles(s, mode == 2 ? coarse_buckets : num_buckets)) != PP_OK) return st;
+    PP_HIP(s->ctx, launch_bucket_offsets(s->ctx->stream, len, num_buckets, mode, delta, coarse_buckets, fold,
+                                         out_start, s->d_stats + 3, s->d_tiles));
+    return PP_OK;
+}
+
+// len and max_bucket of the current layout, read back (one synchronization).
+pp_status count_layout(pp_sum* s, int which, uint32_t num_buckets) {
+    pp_status st;
+    if ((st = clear_stats(s)) != PP_OK) return st;
+    PP_HIP(s->ctx, launch_reduce_lens(s->ctx->stream, s->bk[which].len, num_buckets, s->d_stats, nullptr));
+    if ((st = read_stats(s)) != PP_OK) return st;
+    s->len = s->h_stats[0];
+    s->max_bucket = (uint32_t)s->h_stats[1];
+    return PP_OK;
+}
+
 // ---- device rebucketing ------------------------------------------------------------------------------------------
 
 pp_status refine_once(pp_sum* s) {
@@ -324,8 +359,7 @@ pp_status refine_once(pp_sum* s) {
         return launch_refine<W, T>(s->ctx->stream, a);
     });
     PP_HIP(s->ctx, e);
-    if ((st = read_stats(s)) != PP_OK) return st;
-    s->max_bucket = (uint32_t)s->h_stats[1];
+    if ((st = count_layout(s, out, 2 * B)) != PP_OK) return st;
     s->device_bits += 1;
     s->cur = out;
     return PP_OK;
@@ -337,9 +371,7 @@ pp_status coarsen_to(pp_sum* s, int bits) {
     pp_status st;
     if ((st = ensure_columns(s, out, std::max<uint64_t>(s->len, 1))) != PP_OK) return st;
     if ((st = ensure_buckets(s, out, CB)) != PP_OK) return st;
-    if ((st = clear_stats(s)) != PP_OK) return st;
-    PP_HIP(s->ctx, launch_bucket_offsets(s->ctx->stream, s->bk[in].len, B, 2, 0, CB, B / CB, s->bk[out].start,
-                                         s->d_stats + 3));
+    if ((st = offsets(s, s->bk[in].len, B, 2, 0, CB, B / CB, s->bk[out].start)) != PP_OK) return st;
     hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
         constexpr int W = decltype(Wc)::value;
         using T = decltype(Tv);
@@ -362,8 +394,7 @@ pp_status coarsen_to(pp_sum* s, int bits) {
         return launch_coarsen<W, T>(s->ctx->stream, a);
     });
     PP_HIP(s->ctx, e);
-    if ((st = read_stats(s)) != PP_OK) return st;
-    s->max_bucket = (uint32_t)s->h_stats[1];
+    if ((st = count_layout(s, out, CB)) != PP_OK) return st;
     s->device_bits = bits;
     s->cur = out;
     return PP_OK;
@@ -486,8 +517,7 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
     if ((st = ensure_buckets(dst, out, B)) != PP_OK) return st;
     if ((st = clear_stats(dst)) != PP_OK) return st;
     const uint32_t delta = bucket_of_host(src, ch.gx, ch.gz, src->device_bits);
-    PP_HIP(ctx, launch_bucket_offsets(ctx->stream, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start,
-                                      dst->d_stats + 3));
+    if ((st = offsets(dst, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start)) != PP_OK) return st;
     uint32_t lcap = delta ? 2u * src->max_bucket : src->max_bucket;
     lcap = std::max<uint32_t>(64, (lcap + 63) & ~63u);
     if (lcap > kMaxPairRows) lcap = kMaxPairRows;
@@ -529,7 +559,6 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         a.clifford = p.clifford ? 1 : 0;
         a.keep = keep;
         a.stats = dst->d_stats;
-        a.prof = ctx->profiling ? ctx->d_prof : nullptr;
         return launch_rotation_join<W, T>(ctx->stream, a, cosets);
     });
     PP_HIP(ctx, e);
@@ -538,6 +567,8 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         ctx->events.push_back({ev0, ev1});
         ctx->prof_rows_in += src->len;
     }
+    PP_HIP(ctx, launch_reduce_lens(ctx->stream, dst->bk[out].len, B, dst->d_stats,
+                                   ctx->profiling ? ctx->d_prof : nullptr));
     return PP_OK;
 }
 
@@ -568,10 +599,7 @@ pp_status device_layer(pp_sum* s, const pp_channel& ch, const Prepared& p, const
             return launch_rescale<W, T>(ctx->stream, a, B);
         });
         PP_HIP(ctx, e);
-        if ((st = read_stats(s)) != PP_OK) return st;
-        s->len = s->h_stats[0];
-        s->max_bucket = (uint32_t)s->h_stats[1];
-        return PP_OK;
+        return count_layout(s, s->cur, B);
     }
     const int out = 1 - s->cur;
     if ((st = launch_join(s, s, out, ch, p, keep)) != PP_OK) return st;
@@ -770,6 +798,7 @@ pp_status pp_sum_free(pp_sum* s) {
     if (s->h_stats) (void)hipHostFree(s->h_stats);
     if (s->d_topn) (void)hipFree(s->d_topn);
     if (s->d_hist) (void)hipFree(s->d_hist);
+    if (s->d_tiles) (void)hipFree(s->d_tiles);
     delete s;
     return PP_OK;
 }
@@ -873,7 +902,11 @@ pp_status pp_sum_download(pp_sum* s, uint64_t* x, uint64_t* z, void* coeff) {
         cleanup();
         return st;
     }
-    hipError_t e = launch_bucket_offsets(ctx->stream, s->bk[s->cur].len, B, 0, 0, 0, 0, d_dense_start, s->d_stats + 3);
+    if ((st = offsets(s, s->bk[s->cur].len, B, 0, 0, 0, 0, d_dense_start)) != PP_OK) {
+        cleanup();
+        return st;
+    }
+    hipError_t e = hipSuccess;
     if (e == hipSuccess)
         e = dispatch(s, [&](auto Wc, auto Tv) {
             constexpr int WW = decltype(Wc)::value;

@@ -35,8 +35,7 @@ struct JoinArgs {
     T sin_t;
     int32_t clifford;
     KeepProgram keep;
-    unsigned long long* stats;  // [0] += output rows, [1] = max output bucket, [2] |= LDS overflow
-    unsigned long long* prof;   // null, or [0] += output rows across launches (measurement)
+    unsigned long long* stats;  // [2] |= LDS overflow (never expected: the host sizes lcap for the largest pair)
 };
 
 template <int W, typename T>
@@ -126,9 +125,14 @@ struct ExportArgs {
     uint32_t* out_bucket;
 };
 
+// Exclusive scan of per-bucket sizes into out_start (three launches); tile_sums holds offsets_tiles(n) words.
+uint32_t offsets_tiles(uint32_t out_n);
 hipError_t launch_bucket_offsets(hipStream_t st, const uint32_t* len, uint32_t num_buckets, uint32_t mode,
                                  uint32_t delta, uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start,
-                                 unsigned long long* total);
+                                 unsigned long long* total, uint64_t* tile_sums);
+// out[0] += total rows, out[1] = max(out[1], longest bucket); prof[0] += total rows when prof is given.
+hipError_t launch_reduce_lens(hipStream_t st, const uint32_t* len, uint32_t num_buckets, unsigned long long* out,
+                              unsigned long long* prof);
 
 template <int W, typename T>
 size_t join_lds_bytes(uint32_t lcap, uint32_t tcap_bits);

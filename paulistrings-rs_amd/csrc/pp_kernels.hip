@@ -12,52 +12,141 @@
 namespace pp {
 
 // ---------------------------------------------------------------------------------------------------------------
-// Bucket offsets: one workgroup of 1024 threads scans the per-bucket sizes of the next layout.
+// Bucket offsets of the next layout: a three-kernel exclusive scan over per-bucket sizes (tiles of
+// kScanTile buckets: tile sums, one workgroup scans the tile sums, then each tile scans itself).
+// mode 0: size = len[b]; mode 1: rotation output room len[b] + len[b ^ delta] (2 len[b] when delta = 0);
+// mode 2: coarsened bucket b < coarse_buckets gathers b + e * coarse_buckets for e < fold.
 // ---------------------------------------------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(1024) k_bucket_offsets(const uint32_t* __restrict__ len, uint32_t num_buckets,
-                                                          uint32_t mode, uint32_t delta, uint32_t coarse_buckets,
-                                                          uint32_t fold, uint64_t* __restrict__ out_start,
-                                                          unsigned long long* __restrict__ total) {
-    // mode 0: size = len[b]; mode 1: rotation output room len[b] + len[b ^ delta] (2 len[b] when delta = 0);
-    // mode 2: coarsened bucket b < coarse_buckets gathers b + e * coarse_buckets for e < fold.
-    __shared__ uint64_t wave_sums[16];
-    const uint32_t out_n = mode == 2 ? coarse_buckets : num_buckets;
-    const uint32_t per = (out_n + 1023) / 1024;
-    const uint32_t lo = min(out_n, threadIdx.x * per), hi = min(out_n, lo + per);
-    auto size_of = [&](uint32_t b) -> uint64_t {
-        if (mode == 0) return len[b];
-        if (mode == 1) return delta ? (uint64_t)len[b] + len[b ^ delta] : 2ull * len[b];
-        uint64_t s = 0;
-        for (uint32_t e = 0; e < fold; ++e) s += len[b + e * coarse_buckets];
-        return s;
-    };
-    uint64_t mine = 0;
-    for (uint32_t b = lo; b < hi; ++b) mine += size_of(b);
-    // inclusive scan of `mine` across the block
-    uint64_t v = mine;
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+constexpr uint32_t kScanItems = 8;
+constexpr uint32_t kScanTile = kBlock * kScanItems;
+
+struct OffsetsArgs {
+    const uint32_t* len;
+    uint32_t out_n;
+    uint32_t mode;
+    uint32_t delta;
+    uint32_t coarse_buckets;
+    uint32_t fold;
+    uint64_t* out_start;
+    uint64_t* tile_sums;
+    unsigned long long* total;
+};
+
+__device__ __forceinline__ uint64_t bucket_size(const OffsetsArgs& a, uint32_t b) {
+    if (a.mode == 0) return a.len[b];
+    if (a.mode == 1) return a.delta ? (uint64_t)a.len[b] + a.len[b ^ a.delta] : 2ull * a.len[b];
+    uint64_t s = 0;
+    for (uint32_t e = 0; e < a.fold; ++e) s += a.len[b + e * a.coarse_buckets];
+    return s;
+}
+
+// Exclusive scan of one u64 per thread across the workgroup; returns the block total in *total.
+__device__ __forceinline__ uint64_t block_exclusive_scan_u64(uint64_t v, uint64_t* wave_sums, uint64_t* total) {
+    const uint32_t lane = lane_id(), wave = wave_id();
+    uint64_t inc = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const uint64_t o = __shfl_up(v, d, 64);
-        if (lane >= (uint32_t)d) v += o;
+        const uint64_t o = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += o;
     }
-    if (lane == 63) wave_sums[wave] = v;
+    if (lane == 63) wave_sums[wave] = inc;
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const uint64_t s = wave_sums[w];
+        before += (uint32_t)w < wave ? s : 0ull;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_offsets_tiles(const OffsetsArgs a) {
+    __shared__ uint64_t wave_sums[kWaves];
+    const uint32_t lo = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint64_t mine = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kScanItems; ++i)
+        if (lo + i < a.out_n) mine += bucket_size(a, lo + i);
+    uint64_t total;
+    block_exclusive_scan_u64(mine, wave_sums, &total);
+    if (threadIdx.x == 0) a.tile_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kBlock) k_offsets_top(const OffsetsArgs a, uint32_t tiles) {
+    __shared__ uint64_t wave_sums[kWaves];
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < tiles; base += kBlock) {
+        const uint32_t t = base + threadIdx.x;
+        const uint64_t v = t < tiles ? a.tile_sums[t] : 0ull;
+        uint64_t total;
+        const uint64_t ex = block_exclusive_scan_u64(v, wave_sums, &total);
+        if (t < tiles) a.tile_sums[t] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && a.total) *a.total = carry;
+}
+
+__global__ void __launch_bounds__(kBlock) k_offsets_final(const OffsetsArgs a) {
+    __shared__ uint64_t wave_sums[kWaves];
+    const uint32_t lo = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint64_t sz[kScanItems];
+    uint64_t mine = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kScanItems; ++i) {
+        sz[i] = lo + i < a.out_n ? bucket_size(a, lo + i) : 0ull;
+        mine += sz[i];
+    }
+    uint64_t total;
+    uint64_t base = a.tile_sums[blockIdx.x] + block_exclusive_scan_u64(mine, wave_sums, &total);
+#pragma unroll
+    for (uint32_t i = 0; i < kScanItems; ++i)
+        if (lo + i < a.out_n) {
+            a.out_start[lo + i] = base;
+            base += sz[i];
+        }
+}
+
+// Total and longest bucket of a layout, with a handful of atomics per launch (one per workgroup), and the
+// total added to a measurement counter when one is given.
+__global__ void __launch_bounds__(kBlock) k_reduce_lens(const uint32_t* __restrict__ len, uint32_t num_buckets,
+                                                        unsigned long long* __restrict__ out,
+                                                        unsigned long long* __restrict__ prof) {
+    __shared__ uint64_t wave_sums[kWaves];
+    __shared__ uint32_t wave_max[kWaves];
+    const uint32_t lo = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint64_t s = 0;
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kScanItems; ++i)
+        if (lo + i < num_buckets) {
+            const uint32_t v = len[lo + i];
+            s += v;
+            m = max(m, v);
+        }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        s += __shfl_down(s, d, 64);
+        m = max(m, (uint32_t)__shfl_down((int)m, d, 64));
+    }
+    if (lane_id() == 0) {
+        wave_sums[wave_id()] = s;
+        wave_max[wave_id()] = m;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t run = 0;
-        for (int w = 0; w < 16; ++w) {
-            const uint64_t t = wave_sums[w];
-            wave_sums[w] = run;
-            run += t;
+        uint64_t ts = 0;
+        uint32_t tm = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            ts += wave_sums[w];
+            tm = max(tm, wave_max[w]);
         }
-        *total = run;
-    }
-    __syncthreads();
-    uint64_t base = wave_sums[wave] + v - mine;
-    for (uint32_t b = lo; b < hi; ++b) {
-        out_start[b] = base;
-        base += size_of(b);
+        if (ts) atomicAdd(&out[0], (unsigned long long)ts);
+        if (tm) atomicMax(&out[1], (unsigned long long)tm);
+        if (prof && ts) atomicAdd(&prof[0], (unsigned long long)ts);
     }
 }
 
@@ -245,15 +334,11 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
             cur1 += tot[1] + tot[3];
         }
     }
+    // The counts stay per bucket: a total through one global atomic per workgroup would serialize ~10^5
+    // same-address atomics per launch, so k_reduce_lens sums them afterwards.
     if (threadIdx.x == 0) {
         a.olen[b0] = cur0;
         if (pair) a.olen[b1] = cur1;
-        const uint32_t m = max(cur0, cur1);
-        if (cur0 + cur1) {
-            atomicAdd(&a.stats[0], (unsigned long long)(cur0 + cur1));
-            if (a.prof) atomicAdd(&a.prof[0], (unsigned long long)(cur0 + cur1));
-        }
-        atomicMax(&a.stats[1], (unsigned long long)m);
     }
 }
 
@@ -299,11 +384,7 @@ __global__ void __launch_bounds__(kBlock) k_rescale(const RescaleArgs<W, T> a) {
         }
         cur += tot[0];
     }
-    if (threadIdx.x == 0) {
-        a.len[b] = cur;
-        if (cur) atomicAdd(&a.stats[0], (unsigned long long)cur);
-        atomicMax(&a.stats[1], (unsigned long long)cur);
-    }
+    if (threadIdx.x == 0) a.len[b] = cur;
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -344,7 +425,6 @@ __global__ void __launch_bounds__(kBlock) k_refine(const RefineArgs<W, T> a) {
         a.olen[b] = cur0;
         a.ostart[b + a.num_buckets] = s + cur0;
         a.olen[b + a.num_buckets] = cur1;
-        atomicMax(&a.stats[1], (unsigned long long)max(cur0, cur1));
     }
 }
 
@@ -366,7 +446,6 @@ __global__ void __launch_bounds__(kBlock) k_coarsen(const CoarsenArgs<W, T> a) {
         uint32_t total = 0;
         for (uint32_t e = cb; e < a.num_buckets; e += a.coarse_buckets) total += a.len[e];
         a.olen[cb] = total;
-        atomicMax(&a.stats[1], (unsigned long long)total);
     }
 }
 
@@ -435,11 +514,34 @@ __global__ void __launch_bounds__(kBlock) k_export(const ExportArgs<W, T> a) {
 // Launchers
 // ---------------------------------------------------------------------------------------------------------------
 
+uint32_t offsets_tiles(uint32_t out_n) { return (out_n + kScanTile - 1) / kScanTile; }
+
 hipError_t launch_bucket_offsets(hipStream_t st, const uint32_t* len, uint32_t num_buckets, uint32_t mode,
                                  uint32_t delta, uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start,
-                                 unsigned long long* total) {
-    hipLaunchKernelGGL(k_bucket_offsets, dim3(1), dim3(1024), 0, st, len, num_buckets, mode, delta, coarse_buckets,
-                       fold, out_start, total);
+                                 unsigned long long* total, uint64_t* tile_sums) {
+    OffsetsArgs a{};
+    a.len = len;
+    a.out_n = mode == 2 ? coarse_buckets : num_buckets;
+    a.mode = mode;
+    a.delta = delta;
+    a.coarse_buckets = coarse_buckets;
+    a.fold = fold;
+    a.out_start = out_start;
+    a.tile_sums = tile_sums;
+    a.total = total;
+    const uint32_t tiles = offsets_tiles(a.out_n);
+    if (tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_offsets_tiles, dim3(tiles), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_offsets_top, dim3(1), dim3(kBlock), 0, st, a, tiles);
+    hipLaunchKernelGGL(k_offsets_final, dim3(tiles), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_lens(hipStream_t st, const uint32_t* len, uint32_t num_buckets, unsigned long long* out,
+                              unsigned long long* prof) {
+    const uint32_t blocks = offsets_tiles(num_buckets);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_lens, dim3(blocks), dim3(kBlock), 0, st, len, num_buckets, out, prof);
     return hipGetLastError();
 }
 

@@ -143,11 +143,7 @@ __global__ void __launch_bounds__(kBlock) k_topn_retain(const TopnRetainArgs<W, 
         }
         cur += tot[0];
     }
-    if (threadIdx.x == 0) {
-        a.len[b] = cur;
-        if (cur) atomicAdd(&a.stats[0], (unsigned long long)cur);
-        atomicMax(&a.stats[1], (unsigned long long)cur);
-    }
+    if (threadIdx.x == 0) a.len[b] = cur;
 }
 
 __global__ void k_mark_positions(const uint64_t* pos, uint64_t n, uint8_t* flags) {

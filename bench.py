@@ -19,6 +19,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -188,6 +189,9 @@ def main() -> int:
     ap.add_argument("--cpu-log2n", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (the reference for the HBM "
+                         "traffic counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not a configuration it measures "
@@ -195,6 +199,18 @@ def main() -> int:
         return 2
 
     import numpy as np  # noqa: F401
+
+    if args.calibrate_copy:
+        import torch
+
+        n = 1 << 27  # 2^27 float64 = 1 GiB
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8, "reps": 3}), flush=True)
+        return 0
 
     channels = dense_layer_channels(SEED)
     log(f"generating 2^{args.log2n} saturated keys ({args.dtype})")
@@ -214,9 +230,11 @@ def main() -> int:
         cpu = cpu_baseline(channels, args.dtype, args.cpu_log2n, args.cpu_seconds)
 
     traffic = None
-    prof = ROOT / "profiles" / "rotation_join_saturated_layer_traffic.json"
-    if prof.exists():
-        traffic = json.loads(prof.read_text()).get(f"bytes_per_launch_{args.dtype}")
+    if args.log2n == 24:
+        prof = ROOT / "profiles" / f"rotation_join_saturated_layer_2e24_{args.dtype}_traffic.md"
+        if prof.exists():
+            m = re.search(r"^traffic_bytes_per_launch: (\d+)$", prof.read_text(), re.M)
+            traffic = int(m.group(1)) if m else None
     out = {
         "metric": METRIC,
         "value": value,

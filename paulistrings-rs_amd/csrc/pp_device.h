@@ -13,6 +13,10 @@ namespace pp {
 constexpr int kBlock = 256;  // threads per workgroup: four wave64s
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxKeepNodes = 15;
+// Bucket segments of a resident layout start on a multiple of kSegAlign rows (128 bytes of every 8-byte column),
+// so that no cache line holds the end of one bucket and the start of the next: two workgroups, possibly on
+// different XCDs with their own L2, would each move that line.
+constexpr uint32_t kSegAlign = 16;
 constexpr int kMaxHashRows = 31;
 
 enum KeepOp : int32_t { KEEP_TRUE = 0, KEEP_COEFF = 1, KEEP_WEIGHT = 2, KEEP_AND = 3, KEEP_OR = 4 };
@@ -82,11 +86,53 @@ __device__ __forceinline__ uint32_t product_phase(const Key<W>& k, const Key<W>&
     return d & 3u;
 }
 
+// The generator's support: bit w set when word w of g has an x or z bit.  Outside it c = e = 0, where the
+// |a & b| and |(a ^ c) & (b ^ e)| terms of the phase cancel and x.gz ^ z.gx vanishes, so only the support's
+// words are read.  The mask is uniform (a kernel argument), so the skipped words cost scalar branches only.
+template <int W>
+__device__ __forceinline__ uint32_t support_words(const Key<W>& g) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) m |= ((g.x[w] | g.z[w]) != 0 ? 1u : 0u) << w;
+    return m;
+}
+
+template <int W>
+__device__ __forceinline__ bool anticommutes_on(const Key<W>& k, const Key<W>& g, uint32_t supp) {
+    uint32_t f = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (supp & (1u << w)) f += __popcll((k.x[w] & g.z[w]) ^ (k.z[w] & g.x[w]));
+    return f & 1u;
+}
+
+// product_phase over the support words; gphase = |gx & gz| summed over every word (a per-launch constant).
+template <int W>
+__device__ __forceinline__ uint32_t product_phase_on(const Key<W>& k, const Key<W>& g, uint32_t supp,
+                                                     uint32_t gphase) {
+    uint32_t d = gphase;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        if (supp & (1u << w)) {
+            const uint64_t a = k.x[w], b = k.z[w], c = g.x[w], e = g.z[w];
+            d += 2u * __popcll(b & c) + __popcll(a & b) - __popcll((a ^ c) & (b ^ e));
+        }
+    }
+    return d & 3u;
+}
+
 // The partner amplitude of an anticommuting k: i * sin * (k g) carries the real phase i^(1 + d),
 // a sign (channel/rotation.rs:57-65).
 template <int W, typename T>
 __device__ __forceinline__ T partner_amplitude(const Key<W>& k, const Key<W>& g, T sin_t) {
     const uint32_t total = (1u + product_phase<W>(k, g)) & 3u;
+    return (total & 2u) ? -sin_t : sin_t;
+}
+
+template <int W, typename T>
+__device__ __forceinline__ T partner_amplitude_on(const Key<W>& k, const Key<W>& g, uint32_t supp,
+                                                  uint32_t gphase, T sin_t) {
+    const uint32_t total = (1u + product_phase_on<W>(k, g, supp, gphase)) & 3u;
     return (total & 2u) ? -sin_t : sin_t;
 }
 
@@ -124,6 +170,7 @@ __device__ __forceinline__ bool survives(const KeepProgram& p, const Key<W>& k, 
     if (!(c != T(0))) return false;
     if (p.n == 0) return true;
     const double v = (double)c;
+    if (p.n == 1 && p.op[0] == KEEP_COEFF) return p.disabled[0] || v * v > p.eps2[0];  // the common policy
     // The operand stack is a bit string (top = bit 0): a private array indexed by a loop counter would
     // live in scratch memory.
     uint32_t stack = 0;
@@ -163,24 +210,26 @@ __device__ __forceinline__ uint32_t bucket_of(const Key<W>& k, const uint64_t* _
 }
 
 // A 32-bit mix of the key for the LDS join table; any function works, a good one keeps probes short.
+// The LDS table's hash: a GF(2)-linear fold of the key's 32-bit halves, each rotated by its own amount (distinct
+// for the x and z halves of one qubit range and for neighbouring ranges, so that one- and two-qubit differences
+// do not cancel), then a multiplicative finish.  Linearity gives fold(k ^ g) = fold(k) ^ fold(g): a partner's fold
+// costs one XOR.
 template <int W>
-__device__ __forceinline__ uint32_t key_hash(const Key<W>& k) {
-    uint32_t h = 0x9E3779B9u;
+__device__ __forceinline__ uint32_t key_fold(const Key<W>& k) {
+    uint32_t f = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        uint32_t a = (uint32_t)k.x[w] ^ (uint32_t)(k.x[w] >> 32) * 0x85EBCA77u;
-        uint32_t b = (uint32_t)k.z[w] ^ (uint32_t)(k.z[w] >> 32) * 0xC2B2AE3Du;
-        h = (h ^ a) * 0x27D4EB2Fu;
-        h = (h ^ (h >> 15)) ^ b;
-        h *= 0x165667B1u;
+        const uint32_t r = 4u * (uint32_t)w;
+        f ^= __builtin_rotateleft32((uint32_t)k.x[w], r);
+        f ^= __builtin_rotateleft32((uint32_t)(k.x[w] >> 32), r + 8u);
+        f ^= __builtin_rotateleft32((uint32_t)k.z[w], r + 16u);
+        f ^= __builtin_rotateleft32((uint32_t)(k.z[w] >> 32), r + 24u);
     }
-    h ^= h >> 16;
-    h *= 0x85EBCA6Bu;
-    h ^= h >> 13;
-    h *= 0xC2B2AE35u;
-    h ^= h >> 16;
-    return h;
+    return f;
 }
+
+// Slot mix of a fold: the table index is its top bits, the 16-bit tag its low half.
+__device__ __forceinline__ uint32_t fold_mix(uint32_t f) { return f * 0x9E3779B1u; }
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 __device__ __forceinline__ uint32_t wave_id() { return threadIdx.x >> 6; }

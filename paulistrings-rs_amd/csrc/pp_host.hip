@@ -32,11 +32,23 @@ constexpr size_t kRunLayers = 8;                  // engine/bucketed/rotation_la
 constexpr int kRunDimension = 3;                  // engine/bucketed/rotation_layer.rs:511
 constexpr uint32_t kMaxPairRows = 1024;           // LDS rows of one join workgroup
 
+// The join's grid (PP_JOIN_GROUPS_PER_CU): one workgroup per pair by default (measured fastest on the saturated
+// layer); 1..32 selects a persistent grid of at most that many resident workgroups per CU, 0 as many as fit.
+uint32_t join_groups_per_cu() {
+    static uint32_t v = [] {
+        const char* e = std::getenv("PP_JOIN_GROUPS_PER_CU");
+        if (!e || std::strcmp(e, "pairs") == 0) return kJoinOnePairPerGroup;
+        const long n = std::strtol(e, nullptr, 10);
+        return n >= 0 && n <= 32 ? (uint32_t)n : kJoinOnePairPerGroup;
+    }();
+    return v;
+}
+
 uint32_t device_bucket_target() {
     static uint32_t v = [] {
         const char* e = std::getenv("PP_DEVICE_BUCKET_LEN");
         long t = e ? std::strtol(e, nullptr, 10) : 0;
-        return (uint32_t)(t >= 16 && t <= 512 ? t : 128);
+        return (uint32_t)(t >= 16 && t <= 512 ? t : 256);
     }();
     return v;
 }
@@ -238,7 +250,8 @@ pp_status ensure_columns(pp_sum* s, int which, uint64_t need) {
     Columns& c = s->col[which];
     if (c.cap >= need && c.x) return PP_OK;
     free_columns(c);
-    const uint64_t cap = std::max<uint64_t>(need + need / 4, 1024);
+    // a multiple of kSegAlign, so that every word column starts as aligned as the first
+    const uint64_t cap = (std::max<uint64_t>(need + need / 4, 1024) + (kSegAlign - 1)) & ~(uint64_t)(kSegAlign - 1);
     pp_status st;
     if ((st = dev_alloc(s->ctx, (void**)&c.x, cap * s->W * 8)) != PP_OK) return st;
     if ((st = dev_alloc(s->ctx, (void**)&c.z, cap * s->W * 8)) != PP_OK) return st;
@@ -369,7 +382,7 @@ pp_status coarsen_to(pp_sum* s, int bits) {
     const int in = s->cur, out = 1 - s->cur;
     const uint32_t B = 1u << s->device_bits, CB = 1u << bits;
     pp_status st;
-    if ((st = ensure_columns(s, out, std::max<uint64_t>(s->len, 1))) != PP_OK) return st;
+    if ((st = ensure_columns(s, out, s->len + (uint64_t)kSegAlign * CB)) != PP_OK) return st;
     if ((st = ensure_buckets(s, out, CB)) != PP_OK) return st;
     if ((st = offsets(s, s->bk[in].len, B, 2, 0, CB, B / CB, s->bk[out].start)) != PP_OK) return st;
     hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
@@ -513,7 +526,7 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
     pp_status st;
     const int in = src->cur;
     const uint32_t B = 1u << src->device_bits;
-    if ((st = ensure_columns(dst, out, 2 * src->len + 1)) != PP_OK) return st;
+    if ((st = ensure_columns(dst, out, 2 * src->len + (uint64_t)kSegAlign * B)) != PP_OK) return st;
     if ((st = ensure_buckets(dst, out, B)) != PP_OK) return st;
     if ((st = clear_stats(dst)) != PP_OK) return st;
     const uint32_t delta = bucket_of_host(src, ch.gx, ch.gz, src->device_bits);
@@ -550,16 +563,20 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         a.hb = delta ? 31u - (uint32_t)__builtin_clz(delta) : 0u;
         a.lcap = lcap;
         a.tcap_bits = tcap_bits;
+        a.num_cosets = cosets;
+        uint32_t gphase = 0;
         for (uint32_t w = 0; w < W; ++w) {
             a.gx[w] = ch.gx[w];
             a.gz[w] = ch.gz[w];
+            gphase += (uint32_t)__builtin_popcountll(ch.gx[w] & ch.gz[w]);
         }
+        a.gphase = gphase & 3u;
         a.cos_t = (T)p.cos;
         a.sin_t = (T)p.sin;
         a.clifford = p.clifford ? 1 : 0;
         a.keep = keep;
         a.stats = dst->d_stats;
-        return launch_rotation_join<W, T>(ctx->stream, a, cosets);
+        return launch_rotation_join<W, T>(ctx->stream, a, join_groups_per_cu());
     });
     PP_HIP(ctx, e);
     if (ctx->profiling) {
@@ -756,11 +773,11 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
     for (uint32_t b = 0; b < B; ++b) {
         start[b] = acc;
         lens[b] = counts[b];
-        acc += counts[b];
+        acc += (counts[b] + (kSegAlign - 1)) & ~(kSegAlign - 1);
         maxb = std::max(maxb, counts[b]);
     }
     s->max_bucket = maxb;
-    if ((st = ensure_columns(s, 0, std::max<uint64_t>(m, 1))) != PP_OK) return bail(st);
+    if ((st = ensure_columns(s, 0, std::max<uint64_t>(acc, 1))) != PP_OK) return bail(st);
     if ((st = ensure_buckets(s, 0, B)) != PP_OK) return bail(st);
     const uint64_t cap = s->col[0].cap;
     std::vector<uint64_t> hx((size_t)width * cap, 0), hz((size_t)width * cap, 0);

@@ -29,6 +29,8 @@ struct JoinArgs {
     uint32_t hb;         // highest set bit of delta
     uint32_t lcap;       // LDS rows: at least the largest pair
     uint32_t tcap_bits;  // LDS table slots: 1 << tcap_bits >= 2 * lcap
+    uint32_t num_cosets; // pairs {b, b ^ delta} (every bucket on its own when delta = 0)
+    uint32_t gphase;     // sum over words of |gx & gz|, the generator's own term of the product phase
     uint64_t gx[W];
     uint64_t gz[W];
     T cos_t;
@@ -136,8 +138,11 @@ hipError_t launch_reduce_lens(hipStream_t st, const uint32_t* len, uint32_t num_
 
 template <int W, typename T>
 size_t join_lds_bytes(uint32_t lcap, uint32_t tcap_bits);
+// groups_per_cu caps the resident workgroups per CU of the persistent grid (0: as many as fit;
+// kJoinOnePairPerGroup: a workgroup per pair, no persistence).
+constexpr uint32_t kJoinOnePairPerGroup = 0xFFFFFFFFu;
 template <int W, typename T>
-hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t num_cosets);
+hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t groups_per_cu);
 template <int W, typename T>
 hipError_t launch_rescale(hipStream_t st, const RescaleArgs<W, T>& a, uint32_t num_buckets);
 template <int W, typename T>

@@ -33,12 +33,17 @@ struct OffsetsArgs {
     unsigned long long* total;
 };
 
+// Mode 0 packs the buckets densely (the export layout); modes 1 (a join's outputs: at most both buckets of the
+// pair) and 2 (coarsening) allocate resident segments, rounded up to kSegAlign rows.
 __device__ __forceinline__ uint64_t bucket_size(const OffsetsArgs& a, uint32_t b) {
     if (a.mode == 0) return a.len[b];
-    if (a.mode == 1) return a.delta ? (uint64_t)a.len[b] + a.len[b ^ a.delta] : 2ull * a.len[b];
     uint64_t s = 0;
-    for (uint32_t e = 0; e < a.fold; ++e) s += a.len[b + e * a.coarse_buckets];
-    return s;
+    if (a.mode == 1) {
+        s = a.delta ? (uint64_t)a.len[b] + a.len[b ^ a.delta] : 2ull * a.len[b];
+    } else {
+        for (uint32_t e = 0; e < a.fold; ++e) s += a.len[b + e * a.coarse_buckets];
+    }
+    return (s + (kSegAlign - 1)) & ~(uint64_t)(kSegAlign - 1);
 }
 
 // Exclusive scan of one u64 per thread across the workgroup; returns the block total in *total.
@@ -157,11 +162,197 @@ __global__ void __launch_bounds__(kBlock) k_reduce_lens(const uint32_t* __restri
 // Per output key there are at most two contributions, own * cos and partner * sin, so each sum is one rounded
 // add and the result is bitwise the reference's whatever the order.
 //
-// Phase A streams both buckets of the pair: a row commuting with g is final (kept iff nonzero and keep_term);
-// an anticommuting row is staged in LDS.  Phase B builds an open-addressing LDS table over the staged keys and
-// walks them: row k's output is own*cos (0 on the Clifford arm) plus, if k ^ g is staged, that row's c * a1;
-// if k ^ g is not staged it is a new key k ^ g with c * a1 in the other bucket.
+// A workgroup joins the bucket pairs {b, b ^ delta} blockIdx.x, blockIdx.x + G, ... (G = gridDim.x), streaming each
+// pair in chunks of kBlock rows.  Phase A (per chunk): a row commuting with g is final (kept iff nonzero and
+// keep_term) and written at once; an anticommuting row is staged in LDS and entered in an open-addressing table.
+// Phase B (after a pair's last chunk) walks the staged rows: row k's output is own*cos (0 on the Clifford arm)
+// plus, if k ^ g is staged, that row's c * a1; if k ^ g is not staged it is a new key k ^ g with c * a1 in the
+// other bucket.
+//
+// The loads of the next chunk (of this pair, or the first of the next pair) are issued before the current chunk
+// is processed, so they stay in flight through its scan, its writes and the pair's phase B; the next pair's
+// metadata is loaded one pair ahead.  Nothing between a load and its use waits on younger loads: phase A and B
+// read LDS only.
 // ---------------------------------------------------------------------------------------------------------------
+
+// Where a pair lives: its two buckets (b1 = b0 and n1 = 0 for a single-bucket coset), their input segments and
+// their output segments.
+struct PairMeta {
+    uint32_t b0, b1, n0, n1;
+    uint64_t s0, s1, o0, o1;
+};
+
+template <int W, typename T>
+__device__ __forceinline__ void pair_buckets(const JoinArgs<W, T>& a, uint32_t p, uint32_t& b0, uint32_t& b1) {
+    if (a.delta) {
+        const uint32_t lo = p & ((1u << a.hb) - 1u);
+        b0 = ((p >> a.hb) << (a.hb + 1u)) | lo;
+        b1 = b0 ^ a.delta;
+    } else {
+        b0 = b1 = p;
+    }
+}
+
+// A pair's metadata in flight: every lane loads len, start and ostart of b0 (even lanes) or b1 (odd lanes), three
+// unconditional vector loads read out of lanes 0 and 1 with v_readlane once they land.  Nothing is computed from
+// the loaded words before then: a select on them, or a branch merging two of the loads, would make the loads
+// issued after them wait.
+struct MetaWords {
+    uint32_t len;
+    uint64_t start, ostart;
+};
+
+template <int W, typename T>
+__device__ __forceinline__ MetaWords load_meta(const JoinArgs<W, T>& a, uint32_t p) {
+    uint32_t b0, b1;
+    pair_buckets(a, p, b0, b1);
+    const uint32_t b = (threadIdx.x & 1u) ? b1 : b0;
+    MetaWords w;
+    w.len = a.len[b];
+    w.start = a.start[b];
+    w.ostart = a.ostart[b];
+    return w;
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <int W, typename T>
+__device__ __forceinline__ PairMeta read_meta(const JoinArgs<W, T>& a, uint32_t p, const MetaWords& w) {
+    PairMeta m;
+    pair_buckets(a, p, m.b0, m.b1);
+    m.n0 = (uint32_t)__builtin_amdgcn_readlane((int)w.len, 0);
+    m.n1 = a.delta ? (uint32_t)__builtin_amdgcn_readlane((int)w.len, 1) : 0u;
+    m.s0 = readlane_u64(w.start, 0);
+    m.s1 = readlane_u64(w.start, 1);
+    m.o0 = readlane_u64(w.ostart, 0);
+    m.o1 = readlane_u64(w.ostart, 1);
+    return m;
+}
+
+template <int W, typename T>
+struct ChunkRow {
+    Key<W> k;
+    T c;
+};
+
+// Row r of a pair (bucket b0's rows, then b1's); rows at or past n are not loaded.
+template <int W, typename T>
+__device__ __forceinline__ void load_chunk_row(const JoinArgs<W, T>& a, const PairMeta& m, uint32_t r,
+                                               ChunkRow<W, T>& row) {
+    if (r < m.n0 + m.n1) {
+        const uint64_t i = r < m.n0 ? m.s0 + r : m.s1 + (r - m.n0);
+        row.k = load_key<W>(a.x, a.z, a.cap, i);
+        row.c = a.c[i];
+    }
+}
+
+__device__ __forceinline__ uint32_t table_bits(uint32_t n) {
+    uint32_t t = 1;
+    while ((1u << t) < 2u * n) ++t;
+    return t;
+}
+
+// Phase B over staged rows lb .. lb + kBlock - 1 of a pair (see k_rotation_join).
+template <int W, typename T>
+struct Staged {
+    uint64_t* x;  // [W][lcap]
+    uint64_t* z;  // [W][lcap]
+    T* c;
+    uint32_t* fold;
+    uint8_t* side;  // 0: the row came from b0, 1: from b1
+    uint32_t* table;
+    uint32_t lcap;
+};
+
+template <int W, typename T>
+__device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const PairMeta& m, const Key<W>& g,
+                                                 uint32_t supp, uint32_t fold_g, uint32_t tbits, uint32_t nl,
+                                                 uint32_t lb, const Staged<W, T>& st, uint32_t& cur0,
+                                                 uint32_t& cur1, uint32_t* scan_scratch) {
+    const bool pair = a.delta != 0;
+    const uint32_t lcap = st.lcap;
+    const uint64_t* Lx = st.x;
+    const uint64_t* Lz = st.z;
+    const T* Lc = st.c;
+    const uint32_t* Lf = st.fold;
+    const uint8_t* Lm = st.side;
+    const uint32_t* table = st.table;
+    const uint32_t tmask = (1u << tbits) - 1u;
+    const uint32_t i = lb + threadIdx.x;
+    const bool valid = i < nl;
+    Key<W> k, q;
+    T own = T(0), fresh = T(0);
+    uint32_t j = 0;
+    bool keep_own = false, keep_new = false;
+    if (valid) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            k.x[w] = Lx[w * lcap + i];
+            k.z[w] = Lz[w * lcap + i];
+        }
+        const T c = Lc[i];
+        j = Lm[i];
+        q = key_xor<W>(k, g);
+        const uint32_t h = fold_mix(Lf[i] ^ fold_g);
+        const uint32_t tag = h << 16;
+        uint32_t s = h >> (32u - tbits);
+        int found = -1;
+        for (;;) {
+            const uint32_t e = table[s];
+            if (e == 0u) break;
+            if ((e & 0xFFFF0000u) == tag) {
+                const uint32_t idx = (e & 0xFFFFu) - 1u;
+                bool eq = true;
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    eq = eq && (Lx[w * lcap + idx] == q.x[w]) && (Lz[w * lcap + idx] == q.z[w]);
+                if (eq) {
+                    found = (int)idx;
+                    break;
+                }
+            }
+            s = (s + 1u) & tmask;
+        }
+        // one phase evaluation: of the partner q when it is staged (its a1 lands on k), else of k
+        // (whose a1 makes the new key q)
+        const T a1 = partner_amplitude_on<W, T>(found >= 0 ? q : k, g, supp, a.gphase, a.sin_t);
+        own = a.clifford ? T(0) : c * a.cos_t;
+        if (found >= 0) own = own + Lc[found] * a1;
+        keep_own = survives<W, T>(a.keep, k, own);
+        if (found < 0) {
+            fresh = c * a1;
+            keep_new = survives<W, T>(a.keep, q, fresh);
+        }
+    }
+    const uint32_t jn = pair ? (j ^ 1u) : j;
+    bool f[4] = {keep_own && j == 0, keep_own && j == 1, keep_new && jn == 0, keep_new && jn == 1};
+    uint32_t off[4], tot[4];
+    block_flag_scan<4>(f, off, tot, scan_scratch);
+    if (f[0]) {
+        store_key<W>(a.ox, a.oz, a.ocap, m.o0 + cur0 + off[0], k);
+        store_coeff<T>(a.oc, m.o0 + cur0 + off[0], own);
+    }
+    if (f[1]) {
+        store_key<W>(a.ox, a.oz, a.ocap, m.o1 + cur1 + off[1], k);
+        store_coeff<T>(a.oc, m.o1 + cur1 + off[1], own);
+    }
+    if (f[2]) {
+        const uint64_t pos = m.o0 + cur0 + tot[0] + off[2];
+        store_key<W>(a.ox, a.oz, a.ocap, pos, q);
+        store_coeff<T>(a.oc, pos, fresh);
+    }
+    if (f[3]) {
+        const uint64_t pos = m.o1 + cur1 + tot[1] + off[3];
+        store_key<W>(a.ox, a.oz, a.ocap, pos, q);
+        store_coeff<T>(a.oc, pos, fresh);
+    }
+    cur0 += tot[0] + tot[2];
+    cur1 += tot[1] + tot[3];
+}
 
 template <int W, typename T>
 __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a) {
@@ -171,174 +362,132 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
     uint64_t* const Lx = reinterpret_cast<uint64_t*>(lds);   // [W][lcap]
     uint64_t* const Lz = Lx + (size_t)W * lcap;              // [W][lcap]
     T* const Lc = reinterpret_cast<T*>(Lz + (size_t)W * lcap);
-    uint32_t* const table = reinterpret_cast<uint32_t*>(Lc + lcap);
+    uint32_t* const Lf = reinterpret_cast<uint32_t*>(Lc + lcap);  // key_fold of each staged row
+    uint32_t* const table = Lf + lcap;
     uint8_t* const Lm = reinterpret_cast<uint8_t*>(table + (1u << a.tcap_bits));
+    const Staged<W, T> st{Lx, Lz, Lc, Lf, Lm, table, lcap};
 
-    const uint32_t p = blockIdx.x;
-    uint32_t b0, b1;
-    if (a.delta) {
-        const uint32_t lo = p & ((1u << a.hb) - 1u);
-        b0 = ((p >> a.hb) << (a.hb + 1u)) | lo;
-        b1 = b0 ^ a.delta;
-    } else {
-        b0 = b1 = p;
-    }
+    const uint32_t G = gridDim.x, last = a.num_cosets - 1u;
+    uint32_t p = blockIdx.x;
+    if (p > last) return;  // the host launches at most num_cosets workgroups
     const bool pair = a.delta != 0;
-    const uint32_t n0 = a.len[b0];
-    const uint32_t n1 = pair ? a.len[b1] : 0u;
-    const uint32_t n = n0 + n1;
-    const uint64_t s0 = a.start[b0], s1 = a.start[b1];
-    const uint64_t o0 = a.ostart[b0], o1 = a.ostart[b1];
-    if (n > lcap) {  // the host sizes lcap for the largest pair; never taken, guards the LDS bounds
-        if (threadIdx.x == 0) atomicOr(&a.stats[2], 1ull);
-        return;
-    }
     Key<W> g;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         g.x[w] = a.gx[w];
         g.z[w] = a.gz[w];
     }
+    const uint32_t supp = support_words<W>(g);
+    const uint32_t fold_g = key_fold<W>(g);
 
-    uint32_t cur0 = 0, cur1 = 0, nl = 0;
-    // ---- phase A ----
-    for (uint32_t base = 0; base < n; base += kBlock) {
-        const uint32_t r = base + threadIdx.x;
-        const bool valid = r < n;
-        const uint32_t j = r >= n0 ? 1u : 0u;
-        Key<W> k;
-        T c = T(0);
-        bool anti = false, keep = false;
-        if (valid) {
-            const uint64_t row = j ? s1 + (r - n0) : s0 + r;
-            k = load_key<W>(a.x, a.z, a.cap, row);
-            c = a.c[row];
-            anti = anticommutes<W>(k, g);
-            keep = !anti && survives<W, T>(a.keep, k, c);
+    PairMeta m = read_meta(a, p, load_meta(a, p));
+    ChunkRow<W, T> cb;
+    load_chunk_row(a, m, threadIdx.x, cb);
+    MetaWords mw = load_meta(a, min(p + G, last));
+    // Land the first chunk here, so that inside the loop the current chunk is always one copied from the
+    // previous iteration's prefetch (which the copy waited for) and its first use waits on nothing younger.
+    __builtin_amdgcn_s_waitcnt(0);
+    uint32_t base = 0, cur0 = 0, cur1 = 0, nl = 0;
+    // clamped to the allocation: a pair past lcap is refused below, before any insert
+    uint32_t tbits = min(table_bits(m.n0 + m.n1), a.tcap_bits);
+    for (uint32_t i = threadIdx.x; i < (1u << tbits); i += kBlock) table[i] = 0u;
+    for (;;) {
+        const uint32_t n = m.n0 + m.n1;
+        if (n > lcap) {  // the host sizes lcap for the largest pair; never taken, guards the LDS bounds
+            if (threadIdx.x == 0) atomicOr(&a.stats[2], 1ull);
+            return;  // uniform: every wave of the workgroup leaves here
         }
-        bool f[3] = {keep && j == 0, keep && j == 1, anti};
-        uint32_t off[3], tot[3];
-        block_flag_scan<3>(f, off, tot, scan_scratch);
-        if (f[0]) {
-            store_key<W>(a.ox, a.oz, a.ocap, o0 + cur0 + off[0], k);
-            store_coeff<T>(a.oc, o0 + cur0 + off[0], c);
-        }
-        if (f[1]) {
-            store_key<W>(a.ox, a.oz, a.ocap, o1 + cur1 + off[1], k);
-            store_coeff<T>(a.oc, o1 + cur1 + off[1], c);
-        }
-        if (anti) {
-            const uint32_t li = nl + off[2];
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                Lx[w * lcap + li] = k.x[w];
-                Lz[w * lcap + li] = k.z[w];
+        // ---- the next chunk: this pair's, or the first of the next pair ----
+        const bool last_chunk = base + kBlock >= n;
+        bool have_next = true;
+        PairMeta mn = m;
+        uint32_t nbase = base + kBlock;
+        if (last_chunk) {
+            if (p + G <= last) {
+                mn = read_meta(a, p + G, mw);
+                nbase = 0;
+            } else {
+                have_next = false;
             }
-            Lc[li] = c;
-            Lm[li] = (uint8_t)j;
         }
-        cur0 += tot[0];
-        cur1 += tot[1];
-        nl += tot[2];
-    }
+        ChunkRow<W, T> nb;
+        if (have_next) load_chunk_row(a, mn, nbase + threadIdx.x, nb);
+        if (last_chunk && have_next) mw = load_meta(a, min(p + 2u * G, last));
 
-    if (nl != 0) {
-        // ---- the LDS table over the staged keys: slot = (tag << 16) | (index + 1), load at most 1/2 ----
-        uint32_t tbits = 1;
-        while ((1u << tbits) < 2u * nl) ++tbits;
-        const uint32_t tsize = 1u << tbits, tmask = tsize - 1u;
-        for (uint32_t i = threadIdx.x; i < tsize; i += kBlock) table[i] = 0u;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nl; i += kBlock) {
-            Key<W> k;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                k.x[w] = Lx[w * lcap + i];
-                k.z[w] = Lz[w * lcap + i];
-            }
-            const uint32_t h = key_hash<W>(k);
-            const uint32_t entry = (h << 16) | (i + 1u);
-            uint32_t s = h >> (32u - tbits);
-            while (atomicCAS(&table[s], 0u, entry) != 0u) s = (s + 1u) & tmask;
-        }
-        __syncthreads();
-
-        // ---- phase B ----
-        for (uint32_t base = 0; base < nl; base += kBlock) {
-            const uint32_t i = base + threadIdx.x;
-            const bool valid = i < nl;
-            Key<W> k, q;
-            T own = T(0), fresh = T(0);
-            uint32_t j = 0;
-            bool keep_own = false, keep_new = false;
+        // ---- phase A over the current chunk ----
+        {
+            const uint32_t r = base + threadIdx.x;
+            const bool valid = r < n;
+            const uint32_t j = r >= m.n0 ? 1u : 0u;
+            bool anti = false, keep = false;
             if (valid) {
-#pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    k.x[w] = Lx[w * lcap + i];
-                    k.z[w] = Lz[w * lcap + i];
-                }
-                const T c = Lc[i];
-                j = Lm[i];
-                q = key_xor<W>(k, g);
-                const uint32_t h = key_hash<W>(q);
-                const uint32_t tag = h << 16;
-                uint32_t s = h >> (32u - tbits);
-                int found = -1;
-                for (;;) {
-                    const uint32_t e = table[s];
-                    if (e == 0u) break;
-                    if ((e & 0xFFFF0000u) == tag) {
-                        const uint32_t idx = (e & 0xFFFFu) - 1u;
-                        bool eq = true;
-#pragma unroll
-                        for (int w = 0; w < W; ++w)
-                            eq = eq && (Lx[w * lcap + idx] == q.x[w]) && (Lz[w * lcap + idx] == q.z[w]);
-                        if (eq) {
-                            found = (int)idx;
-                            break;
-                        }
-                    }
-                    s = (s + 1u) & tmask;
-                }
-                own = a.clifford ? T(0) : c * a.cos_t;
-                if (found >= 0) own = own + Lc[found] * partner_amplitude<W, T>(q, g, a.sin_t);
-                keep_own = survives<W, T>(a.keep, k, own);
-                if (found < 0) {
-                    fresh = c * partner_amplitude<W, T>(k, g, a.sin_t);
-                    keep_new = survives<W, T>(a.keep, q, fresh);
-                }
+                anti = anticommutes_on<W>(cb.k, g, supp);
+                keep = !anti && survives<W, T>(a.keep, cb.k, cb.c);
             }
-            const uint32_t jn = pair ? (j ^ 1u) : j;
-            bool f[4] = {keep_own && j == 0, keep_own && j == 1, keep_new && jn == 0, keep_new && jn == 1};
-            uint32_t off[4], tot[4];
-            block_flag_scan<4>(f, off, tot, scan_scratch);
+            bool f[3] = {keep && j == 0, keep && j == 1, anti};
+            uint32_t off[3], tot[3];
+            block_flag_scan<3>(f, off, tot, scan_scratch);
             if (f[0]) {
-                store_key<W>(a.ox, a.oz, a.ocap, o0 + cur0 + off[0], k);
-                store_coeff<T>(a.oc, o0 + cur0 + off[0], own);
+                store_key<W>(a.ox, a.oz, a.ocap, m.o0 + cur0 + off[0], cb.k);
+                store_coeff<T>(a.oc, m.o0 + cur0 + off[0], cb.c);
             }
             if (f[1]) {
-                store_key<W>(a.ox, a.oz, a.ocap, o1 + cur1 + off[1], k);
-                store_coeff<T>(a.oc, o1 + cur1 + off[1], own);
+                store_key<W>(a.ox, a.oz, a.ocap, m.o1 + cur1 + off[1], cb.k);
+                store_coeff<T>(a.oc, m.o1 + cur1 + off[1], cb.c);
             }
-            if (f[2]) {
-                const uint64_t pos = o0 + cur0 + tot[0] + off[2];
-                store_key<W>(a.ox, a.oz, a.ocap, pos, q);
-                store_coeff<T>(a.oc, pos, fresh);
+            if (anti) {
+                const uint32_t li = nl + off[2];
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    Lx[w * lcap + li] = cb.k.x[w];
+                    Lz[w * lcap + li] = cb.k.z[w];
+                }
+                Lc[li] = cb.c;
+                Lm[li] = (uint8_t)j;
+                const uint32_t fk = key_fold<W>(cb.k);
+                Lf[li] = fk;
+                // the table: slot = (tag << 16) | (index + 1), load at most 1/2
+                const uint32_t h = fold_mix(fk);
+                const uint32_t entry = (h << 16) | (li + 1u);
+                const uint32_t tmask = (1u << tbits) - 1u;
+                uint32_t s = h >> (32u - tbits);
+                while (atomicCAS(&table[s], 0u, entry) != 0u) s = (s + 1u) & tmask;
             }
-            if (f[3]) {
-                const uint64_t pos = o1 + cur1 + tot[1] + off[3];
-                store_key<W>(a.ox, a.oz, a.ocap, pos, q);
-                store_coeff<T>(a.oc, pos, fresh);
-            }
-            cur0 += tot[0] + tot[2];
-            cur1 += tot[1] + tot[3];
+            cur0 += tot[0];
+            cur1 += tot[1];
+            nl += tot[2];
         }
-    }
-    // The counts stay per bucket: a total through one global atomic per workgroup would serialize ~10^5
-    // same-address atomics per launch, so k_reduce_lens sums them afterwards.
-    if (threadIdx.x == 0) {
-        a.olen[b0] = cur0;
-        if (pair) a.olen[b1] = cur1;
+
+        if (last_chunk) {
+            // ---- phase B over the pair's staged rows ----
+            // One pass in the common case nl <= kBlock: a loop holding stores and no loads would make the
+            // compiler wait for the prefetch before entering it.
+            if (nl != 0) {
+                __syncthreads();  // the staged rows and the table are complete
+                if (nl <= kBlock) {
+                    join_staged_pass<W, T>(a, m, g, supp, fold_g, tbits, nl, 0, st, cur0, cur1, scan_scratch);
+                } else {
+                    for (uint32_t lb = 0; lb < nl; lb += kBlock)
+                        join_staged_pass<W, T>(a, m, g, supp, fold_g, tbits, nl, lb, st, cur0, cur1, scan_scratch);
+                }
+            }
+            // The counts stay per bucket: a total through one global atomic per workgroup would serialize ~10^5
+            // same-address atomics per launch, so k_reduce_lens sums them afterwards.
+            if (threadIdx.x == 0) {
+                a.olen[m.b0] = cur0;
+                if (pair) a.olen[m.b1] = cur1;
+            }
+            if (!have_next) break;
+            // The next pair's table: every lookup above precedes phase B's scan barriers, every insert below
+            // follows the next scan's barrier.
+            p += G;
+            m = mn;
+            cur0 = cur1 = nl = 0;
+            tbits = min(table_bits(m.n0 + m.n1), a.tcap_bits);
+            for (uint32_t i = threadIdx.x; i < (1u << tbits); i += kBlock) table[i] = 0u;
+        }
+        base = nbase;
+        cb = nb;
     }
 }
 
@@ -547,14 +696,33 @@ hipError_t launch_reduce_lens(hipStream_t st, const uint32_t* len, uint32_t num_
 
 template <int W, typename T>
 size_t join_lds_bytes(uint32_t lcap, uint32_t tcap_bits) {
-    return (size_t)lcap * (2 * W * sizeof(uint64_t) + sizeof(T) + 1) + ((size_t)4 << tcap_bits);
+    return (size_t)lcap * (2 * W * sizeof(uint64_t) + sizeof(T) + sizeof(uint32_t) + 1) + ((size_t)4 << tcap_bits);
 }
 
+// A persistent grid: as many workgroups as stay resident at once (at most groups_per_cu per CU when that is
+// nonzero), never more than there are pairs; kJoinOnePairPerGroup launches one workgroup per pair instead.
 template <int W, typename T>
-hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t num_cosets) {
+hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t groups_per_cu) {
+    if (a.num_cosets == 0) return hipSuccess;
     const size_t lds = join_lds_bytes<W, T>(a.lcap, a.tcap_bits);
-    if (num_cosets == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_rotation_join<W, T>), dim3(num_cosets), dim3(kBlock), lds, st, a);
+    static int cus = 0;
+    static int resident[17] = {};  // by lcap / 64 (lcap <= 1024, a multiple of 64)
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    int& per = resident[std::min<uint32_t>(a.lcap >> 6, 16u)];
+    if (per == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rotation_join<W, T>, kBlock, lds) !=
+                         hipSuccess ||
+                     per <= 0))
+        per = 1;
+    const int per_cu = groups_per_cu ? std::min<int>(per, (int)std::min<uint32_t>(groups_per_cu, 1024u)) : per;
+    const uint32_t grid = groups_per_cu == kJoinOnePairPerGroup
+                              ? a.num_cosets
+                              : std::min<uint32_t>(a.num_cosets, (uint32_t)(cus * per_cu));
+    hipLaunchKernelGGL((k_rotation_join<W, T>), dim3(grid), dim3(kBlock), lds, st, a);
     return hipGetLastError();
 }
 

@@ -5,8 +5,7 @@
  * (lkdvos/paulistrings-rs, crates/paulistrings/src/engine/gpu/): a sum resident
  * on one GPU, stepped through Pauli-rotation channels under a built-in
  * truncation policy, read back in the reference's canonical order.  Plain
- * pointers and sizes only; a Rust, ctypes or cgo host binds it as-is
- * (INTEGRATION.md shows the bindings).
+ * pointers and sizes only; a Rust, ctypes or cgo host binds it as-is.
  *
  * Citations are into crates/paulistrings/src/ of the reference.
  *

@@ -21,7 +21,23 @@
  * and the reference's differential oracle naive_apply_layer
  * (test_support/oracle.rs:21-54) as an independent second check.
  *
- * Parity status of each part is recorded in oracle/README.md.
+ * What pins each part (tests/test_oracle.py unless noted):
+ *   ppo_mix64 / ppo_splitmix64 ... the reference's known answer, state 0 -> 0xE220A8397B1DCDAF (rng/tests.rs:3-8)
+ *   ppo_hash_rows ................ at 127 qubits the rows are the attempt-0 draws (the reference's own property,
+ *                                  pauli_sum/hash/tests.rs:209-221), plus full rank, width independence, linearity;
+ *                                  parity unpinned: the rows at qubit counts where rank completion redraws
+ *                                  (never at 127 qubits); they only move bucket ids (TopN tie order, export order)
+ *   ppo_desired_bits / ppo_rebucketed_bits ... hand-computed values of the storage.rs:15-41 formulas
+ *   ppo_sin_cos .................. quarter-turn snaps (theta = -pi/2 gives exactly -YZ); the ulp-level values
+ *                                  are glibc's sincos, the library boundary the reference also crosses:
+ *                                  parity unpinned below the 17-digit golden read-outs
+ *   ppo_propagate (layer) ........ statevector brute force on 4 qubits; the reference's golden kicked-Ising
+ *                                  records (final and peak term counts, <Z62> to 17 digits); the naive layer
+ *   keep programs ................ golden records (threshold, weight cap); an independent predicate over the
+ *                                  unfiltered layer (And, Or, nested, disabled threshold)
+ *   ppo_topn ..................... first_n_by_rank, a stable-sort transcription of test_support/compare.rs:33-55
+ *   ppo_expectation_product ...... statevector read-out on every axis with sign masks; golden <Z62>
+ *   ppo_naive_layer .............. the bucketed engine (two restatements of one semantics, different code)
  */
 #ifndef PP_ORACLE_H
 #define PP_ORACLE_H

@@ -333,3 +333,71 @@ def test_topn_keeps_the_first_ties_in_bucket_order(n):
     want = min(n, len(ec))
     assert len(gc) == want
     assert np.array_equal(gx, ex[keep]) and np.array_equal(gz, ez[keep]) and np.array_equal(gc, ec[keep])
+
+
+# ---- keep programs and read-out against independent predicates ----------------------------------
+
+
+def _weight(x, z):
+    return sum(bin(int(a) | int(b)).count("1") for a, b in zip(x, z))
+
+
+@pytest.mark.parametrize("program", ["or", "and", "nested"])
+def test_keep_programs_equal_a_filter_of_the_unfiltered_layer(program):
+    # merge.rs:232-239: a merged term is emitted iff its sum is nonzero and keep_term holds, so one filtered layer
+    # equals the unfiltered layer with the predicate applied afterwards (builtin.rs:19-59, :470-526).
+    rng = np.random.default_rng(21)
+    x, z = random_terms(rng, 4000, 30, 0.25)
+    c = rng.standard_normal(4000) * np.exp(rng.uniform(-9, 0, 4000))
+    eps, k = 3e-3, 7
+    if program == "or":
+        nodes = [(po.NODE_COEFF, 0, eps), (po.NODE_WEIGHT, k, 0.0), (po.NODE_OR, 0, 0.0)]
+        pred = lambda cc, w: cc * cc > eps * eps or w <= k  # noqa: E731
+    elif program == "and":
+        nodes = [(po.NODE_WEIGHT, k, 0.0), (po.NODE_COEFF, 0, eps), (po.NODE_AND, 0, 0.0)]
+        pred = lambda cc, w: cc * cc > eps * eps and w <= k  # noqa: E731
+    else:  # (coeff(eps) | weight(k)) & weight(k + 3), and a disabled threshold
+        nodes = [(po.NODE_COEFF, 0, eps), (po.NODE_WEIGHT, k, 0.0), (po.NODE_OR, 0, 0.0),
+                 (po.NODE_WEIGHT, k + 3, 0.0), (po.NODE_AND, 0, 0.0), (po.NODE_COEFF, 0, -1.0), (po.NODE_AND, 0, 0.0)]
+        pred = lambda cc, w: (cc * cc > eps * eps or w <= k) and w <= k + 3  # noqa: E731
+    ch = C.Circuit(30).pauli_rotation("ZX", [2, 9], 0.7).channels()
+    ux, uz, uc = po.naive_layer(1, x, z, c, ch, False, po.make_policy())
+    keep = [i for i in range(len(uc)) if uc[i] != 0 and pred(float(uc[i]), _weight(ux[i], uz[i]))]
+    s = po.OracleSum(30, x, z, c)
+    s.propagate(ch, "forward", po.make_policy(nodes))
+    bx, bz, bc, _ = s.export()
+    order = np.lexsort((bz[:, 0], bx[:, 0]))
+    assert np.array_equal(bx[order], ux[keep]) and np.array_equal(bz[order], uz[keep])
+    assert np.array_equal(bc[order], uc[keep])
+
+
+def _product_state(n, axis, negative):
+    """|psi> = tensor of the +1 (or -1 where `negative`) eigenstates of `axis` on every qubit."""
+    vecs = {"z": [np.array([1, 0], complex), np.array([0, 1], complex)],
+            "x": [np.array([1, 1], complex) / math.sqrt(2), np.array([1, -1], complex) / math.sqrt(2)],
+            "y": [np.array([1, 1j], complex) / math.sqrt(2), np.array([1, -1j], complex) / math.sqrt(2)]}[axis]
+    psi = np.ones(1, complex)
+    for q in range(n):
+        psi = np.kron(psi, vecs[(negative >> q) & 1])
+    return psi
+
+
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
+@pytest.mark.parametrize("negative", [0, 0b1011])
+def test_expectation_product_against_the_statevector(axis, negative):
+    # readout/product_state.rs:84-101: <psi| sum c P |psi> for a product state of axis eigenstates
+    n = 4
+    rng = np.random.default_rng(5)
+    terms = {}
+    for _ in range(40):
+        terms[(int(rng.integers(0, 16)), int(rng.integers(0, 16)))] = float(rng.standard_normal())
+    psi = _product_state(n, axis, negative)
+    want = 0.0
+    for (xb, zb), c in terms.items():
+        # pauli_matrix follows the qubit-0-first kron order of _product_state
+        want += c * (psi.conj() @ pauli_matrix(n, xb, zb) @ psi).real
+    x = np.array([[k[0]] for k in terms], np.uint64)
+    z = np.array([[k[1]] for k in terms], np.uint64)
+    s = po.OracleSum(n, x, z, np.array(list(terms.values())))
+    got = s.expectation(axis, negative=[negative])
+    assert abs(got - want) < 1e-12

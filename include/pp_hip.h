@@ -54,8 +54,8 @@ typedef struct {
 
 /* A built-in truncation policy (truncation/builtin.rs, truncation/tree.rs):
  * the per-term filter as a postfix program of at most 15 nodes (the reference's
- * device KeepProgram, engine/gpu/truncation.rs:8-102), and an optional TopN
- * layer pass after every layer (TopN::finalize_layer, builtin.rs:73-129). */
+ * device KeepProgram, engine/gpu/truncation.rs:8-102), and an optional layer
+ * pass after every layer: TopN::finalize_layer (builtin.rs:73-129) or ApproxTopN's (builtin.rs:241-468). */
 typedef enum {
     PP_KEEP_TRUE = 0,
     PP_KEEP_COEFF = 1,  /* CoefficientThreshold(eps): eps < 0 or |c|^2 > eps^2 */
@@ -73,7 +73,7 @@ typedef struct {
 typedef struct {
     int32_t n_nodes; /* 0: keep every nonzero term */
     pp_keep_node nodes[15];
-    int32_t has_topn;
+    int32_t has_topn; /* the layer pass after every layer: 0 none, 1 TopN(topn), 2 ApproxTopN(topn) */
     uint64_t topn;
 } pp_policy;
 
@@ -138,6 +138,11 @@ pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* chann
 /* TopN(n) as a standalone pass (truncation/builtin.rs:73-129): ties at the n-th magnitude are kept
  * in the reference's canonical order. */
 pp_status pp_truncate_topn(pp_sum* sum, uint64_t n);
+
+/* ApproxTopN(n) as a standalone pass (truncation/builtin.rs:241-468, approx_top_n_solo): octaves of |c|^2
+ * kept whole from the top while they fit in n, the boundary octave's first terms in the reference's
+ * canonical order up to n in all; NaN coefficients are never kept once a cut happens. */
+pp_status pp_truncate_approx_topn(pp_sum* sum, uint64_t n);
 
 /* expectation_product (readout/product_state.rs:84-101) in the product state whose qubit q is the
  * +-1 eigenstate of the axis (axis_x, axis_z) bits, -1 where `negative` is set.  W words each. */

@@ -276,6 +276,11 @@ void ppo_topn(ppo_sum* s, uint64_t n) {
     DISPATCH(s, topn_w1_f64(s, n), topn_w1_f32(s, n), topn_w2_f64(s, n), topn_w2_f32(s, n));
 }
 
+void ppo_approx_topn(ppo_sum* s, uint64_t n) {
+    DISPATCH(s, approx_topn_w1_f64(s, n), approx_topn_w1_f32(s, n), approx_topn_w2_f64(s, n),
+             approx_topn_w2_f32(s, n));
+}
+
 static void rebucket_to(ppo_sum* s, int want) {
     DISPATCH(s, rebucket_to_w1_f64(s, want), rebucket_to_w1_f32(s, want), rebucket_to_w2_f64(s, want),
              rebucket_to_w2_f32(s, want));
@@ -386,7 +391,8 @@ static void apply_prepared(ppo_sum* s, const ppo_channel* ch, const prepared_t* 
 /* ---- the layer loop (engine/driver.rs:319-592 at one partition) ------------------------------ */
 
 static size_t run_length_limit(const ppo_policy* pol) {
-    if (pol && pol->has_topn) return (size_t)pol->topn; /* TopN::run_length_limit, builtin.rs:147-149 */
+    /* TopN::run_length_limit and ApproxTopN's (builtin.rs:147-149, :462-464) */
+    if (pol && pol->has_topn) return (size_t)pol->topn;
     return (size_t)-1;                                    /* no layer pass: usize::MAX */
 }
 
@@ -461,7 +467,8 @@ int ppo_propagate(ppo_sum* s, const ppo_channel* channels, size_t n_channels, in
             if (layer_len) layer_len[j] = s->len;
         }
         if (finalizes) {
-            ppo_topn(s, pol->topn);
+            if (pol->has_topn == 2) ppo_approx_topn(s, pol->topn);
+            else ppo_topn(s, pol->topn);
             if (layer_len) layer_len[end - 1] = s->len;
         }
         i = end;

@@ -36,6 +36,8 @@
  *   keep programs ................ golden records (threshold, weight cap); an independent predicate over the
  *                                  unfiltered layer (And, Or, nested, disabled threshold)
  *   ppo_topn ..................... first_n_by_rank, a stable-sort transcription of test_support/compare.rs:33-55
+ *   ppo_approx_topn .............. an independent histogram walk over the canonical export (tests), and the
+ *                                  reference's own cases (truncation/tests.rs ApproxTopN tests, restated)
  *   ppo_expectation_product ...... statevector read-out on every axis with sign masks; golden <Z62>
  *   ppo_naive_layer .............. the bucketed engine (two restatements of one semantics, different code)
  */
@@ -71,7 +73,7 @@ typedef struct {
 typedef struct {
     int32_t n_nodes;          /* 0: keep everything */
     ppo_node nodes[PPO_MAX_NODES];
-    int32_t has_topn;         /* a TopN(topn) layer pass after every layer */
+    int32_t has_topn;         /* a layer pass after every layer: 1 TopN(topn), 2 ApproxTopN(topn) */
     uint64_t topn;
 } ppo_policy;
 
@@ -110,6 +112,9 @@ int ppo_apply_rotation(ppo_sum* s, const ppo_channel* channel, int adjoint, cons
 
 /* TopN::finalize_layer (truncation/builtin.rs:73-129). */
 void ppo_topn(ppo_sum* s, uint64_t n);
+/* ApproxTopN(n)'s layer pass (truncation/builtin.rs:241-468): octaves of |c|^2 kept whole from the top, the
+ * boundary octave's first terms in bucket order up to n. */
+void ppo_approx_topn(ppo_sum* s, uint64_t n);
 
 /* expectation_product (readout/product_state.rs:84-101); buckets summed in ascending order. */
 double ppo_expectation_product(const ppo_sum* s, const uint64_t* axis_x, const uint64_t* axis_z,

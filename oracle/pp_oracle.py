@@ -68,6 +68,7 @@ def lib() -> ctypes.CDLL:
         L.ppo_propagate.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(_Policy), ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, vp]
         L.ppo_topn.argtypes = [vp, ctypes.c_uint64]
+        L.ppo_approx_topn.argtypes = [vp, ctypes.c_uint64]
         L.ppo_expectation_product.restype = ctypes.c_double
         L.ppo_expectation_product.argtypes = [vp, vp, vp, vp]
         L.ppo_naive_layer.restype = ctypes.c_size_t
@@ -93,23 +94,25 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def make_policy(nodes=(), topn=None) -> _Policy:
-    """``nodes``: postfix list of (op, k, eps)."""
+def make_policy(nodes=(), topn=None, approx_topn=None) -> _Policy:
+    """``nodes``: postfix list of (op, k, eps); at most one of ``topn`` (TopN) and ``approx_topn`` (ApproxTopN)."""
     p = _Policy()
     if len(nodes) > MAX_NODES:
         raise ValueError("at most 15 keep nodes")
+    if topn is not None and approx_topn is not None:
+        raise ValueError("one layer pass at most")
     p.n_nodes = len(nodes)
     for i, (op, k, eps) in enumerate(nodes):
         p.nodes[i].op = op
         p.nodes[i].k = k
         p.nodes[i].eps = eps
-    p.has_topn = 0 if topn is None else 1
-    p.topn = 0 if topn is None else int(topn)
+    p.has_topn = 1 if topn is not None else (2 if approx_topn is not None else 0)
+    p.topn = int(topn if topn is not None else (approx_topn if approx_topn is not None else 0))
     return p
 
 
-def coeff_policy(eps=None, max_weight=None, topn=None) -> _Policy:
-    """And(CoefficientThreshold(eps), WeightCutoff(max_weight)) [+ TopN]."""
+def coeff_policy(eps=None, max_weight=None, topn=None, approx_topn=None) -> _Policy:
+    """And(CoefficientThreshold(eps), WeightCutoff(max_weight)) [+ TopN or ApproxTopN]."""
     nodes = []
     if eps is not None:
         nodes.append((NODE_COEFF, 0, float(eps)))
@@ -117,7 +120,7 @@ def coeff_policy(eps=None, max_weight=None, topn=None) -> _Policy:
         nodes.append((NODE_WEIGHT, int(max_weight), 0.0))
         if eps is not None:
             nodes.append((NODE_AND, 0, 0.0))
-    return make_policy(nodes, topn)
+    return make_policy(nodes, topn, approx_topn)
 
 
 class OracleSum:
@@ -185,6 +188,9 @@ class OracleSum:
 
     def topn(self, n: int):
         lib().ppo_topn(self._h, n)
+
+    def approx_topn(self, n: int):
+        lib().ppo_approx_topn(self._h, n)
 
     def expectation(self, axis: str = "z", negative=None) -> float:
         ax = np.zeros(4, np.uint64)

@@ -726,6 +726,77 @@ static void T(topn)(ppo_sum* s, uint64_t n) {
     s->known = 0;
 }
 
+/* ---- ApproxTopN (truncation/builtin.rs:241-468, approx_top_n_solo) -------------------------- */
+
+/* The octave of |c|^2: its exponent bits, norm_sqr().to_bits() >> 52 (builtin.rs:276-279).  A NaN of either
+ * sign lands in octave 2047, as a positive NaN does in the reference (a negative one would index past its
+ * 2048 bins). */
+static unsigned T(octave)(double m) {
+    uint64_t u;
+    memcpy(&u, &m, 8);
+    return (unsigned)((u >> 52) & 0x7FFu);
+}
+
+static void T(approx_topn)(ppo_sum* s, uint64_t n) {
+    size_t total = s->len;
+    if (total <= n) return; /* EdgeDecision::KeepAll: nothing dropped, NaN included */
+    size_t nb = (size_t)1 << s->bits;
+    T(bucket)* bk = (T(bucket)*)s->buckets;
+    if (n == 0) { /* EdgeDecision::Clear */
+        for (size_t b = 0; b < nb; ++b) bk[b].len = 0;
+        s->len = 0;
+        s->known = 0;
+        return;
+    }
+    /* octave_histogram, then octave_edge: walk from the top octave down until one does not fit */
+    size_t hist[2048];
+    memset(hist, 0, sizeof(hist));
+    for (size_t b = 0; b < nb; ++b)
+        for (uint32_t i = 0; i < bk[b].len; ++i) {
+            double v = (double)bk[b].c[i];
+            hist[T(octave)(v * v)]++;
+        }
+    size_t kept_above = 0;
+    int bin = -1;
+    for (int o = 2047; o >= 0; --o) {
+        if (kept_above + hist[o] > n) {
+            bin = o;
+            break;
+        }
+        kept_above += hist[o];
+    }
+    if (bin < 0) return; /* unreachable: more than n terms overflow n at some octave */
+    size_t quota = n - kept_above;
+    /* retain_with_boundary_quota (builtin.rs:208-239): above the boundary octave kept, the first `quota` at it
+     * in bucket order kept, NaN below every octave */
+    size_t len = 0;
+    for (size_t b = 0; b < nb; ++b) {
+        T(bucket)* bb = &bk[b];
+        uint32_t w = 0;
+        for (uint32_t i = 0; i < bb->len; ++i) {
+            double v = (double)bb->c[i];
+            double m = v * v;
+            int side;
+            if (m != m) side = -1;
+            else {
+                unsigned o = T(octave)(m);
+                side = o > (unsigned)bin ? 1 : (o == (unsigned)bin ? 0 : -1);
+            }
+            int at = (side == 0) && quota > 0;
+            quota -= (size_t)at;
+            if (side == 1 || at) {
+                bb->k[w] = bb->k[i];
+                bb->c[w] = bb->c[i];
+                ++w;
+            }
+        }
+        bb->len = w;
+        len += w;
+    }
+    s->len = len;
+    s->known = 0;
+}
+
 /* ---- read-out and export ------------------------------------------------------------------- */
 
 static double T(expectation)(const ppo_sum* s, const uint64_t* ax, const uint64_t* az, const uint64_t* neg) {

@@ -238,32 +238,49 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-// Ordered compaction of NF one-bit flags across the workgroup: ballot + popcount within a wave64, the four
-// wave totals through LDS.  `off[i]` is the thread's rank among set flags i, `tot[i]` the block's count.
-// `scratch` holds NF * kWaves words.  Two barriers; every thread of the block must call it.
+// Ordered compaction of NF one-bit flags across the workgroup: ballot + popcount within a wave64, the wave
+// totals through LDS.  `off[i]` is the thread's rank among set flags i, `tot[i]` the block's count.  The wave
+// totals travel packed, two 16-bit fields per word (a block count is at most kBlock), so the cross-wave sums
+// cost one add per word rather than per flag.  `scratch` holds at least NF * kWaves words.  Two barriers;
+// every thread of the block must call it.
 template <int NF>
 __device__ __forceinline__ void block_flag_scan(const bool (&f)[NF], uint32_t (&off)[NF], uint32_t (&tot)[NF],
                                                 uint32_t* scratch) {
-    const uint32_t lane = lane_id(), wave = wave_id();
+    constexpr int NW = (NF + 1) / 2;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave_id());
     const uint64_t lt = lanemask_lt();
+    uint32_t packed[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) packed[k] = 0;
 #pragma unroll
     for (int i = 0; i < NF; ++i) {
         const uint64_t m = __ballot(f[i]);
         off[i] = __popcll(m & lt);
-        if (lane == 0) scratch[i * kWaves + wave] = __popcll(m);
+        packed[i >> 1] |= (uint32_t)__popcll(m) << (16 * (i & 1));
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) scratch[k * kWaves + wave] = packed[k];
     }
     __syncthreads();
+    uint32_t before[NW], all[NW];
 #pragma unroll
-    for (int i = 0; i < NF; ++i) {
-        uint32_t before = 0, all = 0;
+    for (int k = 0; k < NW; ++k) {
+        before[k] = 0;
+        all[k] = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
-            const uint32_t c = scratch[i * kWaves + w];
-            before += (uint32_t)w < wave ? c : 0u;
-            all += c;
+            const uint32_t c = scratch[k * kWaves + w];
+            if ((uint32_t)w < wave) before[k] += c;
+            all[k] += c;
         }
-        off[i] += before;
-        tot[i] = all;
+    }
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+        const int sh = 16 * (i & 1);
+        off[i] += (before[i >> 1] >> sh) & 0xFFFFu;
+        tot[i] = (all[i >> 1] >> sh) & 0xFFFFu;
     }
     __syncthreads();
 }

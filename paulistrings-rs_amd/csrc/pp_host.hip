@@ -499,6 +499,7 @@ KeepProgram lower_policy(const pp_policy* pol) {
 bool valid_policy(const pp_policy* pol) {
     if (!pol) return true;
     if (pol->n_nodes < 0 || pol->n_nodes > 15) return false;
+    if (pol->has_topn < 0 || pol->has_topn > 2) return false;
     int depth = 0;
     for (int i = 0; i < pol->n_nodes; ++i) {
         const int op = pol->nodes[i].op;
@@ -644,7 +645,8 @@ pp_status apply_one(pp_sum* s, const pp_channel& ch, const Prepared& p, const Ke
     return device_layer(s, ch, p, keep);
 }
 
-pp_status topn_device(pp_sum* s, uint64_t n);  // pp_topn_impl.inc
+pp_status topn_device(pp_sum* s, uint64_t n);         // pp_topn_impl.inc
+pp_status approx_topn_device(pp_sum* s, uint64_t n);  // pp_topn_impl.inc
 
 }  // namespace
 
@@ -1032,7 +1034,8 @@ pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channel
     const bool adjoint = direction == PP_HEISENBERG;
     const KeepProgram keep = lower_policy(policy);
     const bool finalizes = policy && policy->has_topn;
-    const uint64_t limit = finalizes ? policy->topn : UINT64_MAX;  // run_length_limit, builtin.rs:147-149
+    // run_length_limit of TopN and ApproxTopN (builtin.rs:147-149, :462-464)
+    const uint64_t limit = finalizes ? policy->topn : UINT64_MAX;
     auto channel_at = [&](uint64_t i) -> const pp_channel& {
         return channels[adjoint ? n_channels - 1 - i : i];
     };
@@ -1086,7 +1089,8 @@ pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channel
             if (layer_len) layer_len[j] = s->len;
         }
         if (finalizes) {
-            if ((st = pp_truncate_topn(s, policy->topn)) != PP_OK) return st;
+            st = policy->has_topn == 2 ? pp_truncate_approx_topn(s, policy->topn) : pp_truncate_topn(s, policy->topn);
+            if (st != PP_OK) return st;
             if (layer_len) layer_len[end - 1] = s->len;
         }
         i = end;
@@ -1100,6 +1104,14 @@ pp_status pp_truncate_topn(pp_sum* s, uint64_t n) {
     if (rs != PP_OK) return rs;
     if (s->len <= n) return PP_OK;
     return topn_device(s, n);
+}
+
+pp_status pp_truncate_approx_topn(pp_sum* s, uint64_t n) {
+    if (!s) return PP_ERR_INVALID_ARGUMENT;
+    pp_status rs = refresh(s);
+    if (rs != PP_OK) return rs;
+    if (s->len <= n) return PP_OK;  // EdgeDecision::KeepAll
+    return approx_topn_device(s, n);
 }
 
 pp_status pp_expectation_product(pp_sum* s, const uint64_t* axis_x, const uint64_t* axis_z, const uint64_t* negative,

@@ -332,21 +332,14 @@ __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const 
     bool f[4] = {keep_own && j == 0, keep_own && j == 1, keep_new && jn == 0, keep_new && jn == 1};
     uint32_t off[4], tot[4];
     block_flag_scan<4>(f, off, tot, scan_scratch);
-    if (f[0]) {
-        store_key<W>(a.ox, a.oz, a.ocap, m.o0 + cur0 + off[0], k);
-        store_coeff<T>(a.oc, m.o0 + cur0 + off[0], own);
+    // one store sequence per output kind, for either bucket
+    if (keep_own) {
+        const uint64_t pos = j == 0 ? m.o0 + cur0 + off[0] : m.o1 + cur1 + off[1];
+        store_key<W>(a.ox, a.oz, a.ocap, pos, k);
+        store_coeff<T>(a.oc, pos, own);
     }
-    if (f[1]) {
-        store_key<W>(a.ox, a.oz, a.ocap, m.o1 + cur1 + off[1], k);
-        store_coeff<T>(a.oc, m.o1 + cur1 + off[1], own);
-    }
-    if (f[2]) {
-        const uint64_t pos = m.o0 + cur0 + tot[0] + off[2];
-        store_key<W>(a.ox, a.oz, a.ocap, pos, q);
-        store_coeff<T>(a.oc, pos, fresh);
-    }
-    if (f[3]) {
-        const uint64_t pos = m.o1 + cur1 + tot[1] + off[3];
+    if (keep_new) {
+        const uint64_t pos = jn == 0 ? m.o0 + cur0 + tot[0] + off[2] : m.o1 + cur1 + tot[1] + off[3];
         store_key<W>(a.ox, a.oz, a.ocap, pos, q);
         store_coeff<T>(a.oc, pos, fresh);
     }
@@ -427,13 +420,10 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
             bool f[3] = {keep && j == 0, keep && j == 1, anti};
             uint32_t off[3], tot[3];
             block_flag_scan<3>(f, off, tot, scan_scratch);
-            if (f[0]) {
-                store_key<W>(a.ox, a.oz, a.ocap, m.o0 + cur0 + off[0], cb.k);
-                store_coeff<T>(a.oc, m.o0 + cur0 + off[0], cb.c);
-            }
-            if (f[1]) {
-                store_key<W>(a.ox, a.oz, a.ocap, m.o1 + cur1 + off[1], cb.k);
-                store_coeff<T>(a.oc, m.o1 + cur1 + off[1], cb.c);
+            if (keep) {  // one store sequence for either bucket
+                const uint64_t pos = j == 0 ? m.o0 + cur0 + off[0] : m.o1 + cur1 + off[1];
+                store_key<W>(a.ox, a.oz, a.ocap, pos, cb.k);
+                store_coeff<T>(a.oc, pos, cb.c);
             }
             if (anti) {
                 const uint32_t li = nl + off[2];

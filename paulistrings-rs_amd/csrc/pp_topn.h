@@ -81,4 +81,50 @@ template <int W, typename T>
 hipError_t launch_topn_retain(hipStream_t st, const TopnRetainArgs<W, T>& a, uint32_t num_buckets);
 hipError_t launch_mark_positions(hipStream_t st, const uint64_t* pos, uint64_t n, uint8_t* flags);
 
+// ---- ApproxTopN (truncation/builtin.rs:241-468): the cut at an octave of |c|^2 ----------------------------------
+
+constexpr int kOctaveBins = 2048;  // APPROX_BINS, one per f64 exponent (builtin.rs:241-242)
+
+template <typename T>
+struct OctaveHistArgs {
+    const T* c;
+    const uint64_t* start;
+    const uint32_t* len;
+    uint32_t num_buckets;
+    uint32_t buckets_per_block;
+    unsigned long long* hist;  // kOctaveBins
+};
+
+// The boundary octave's terms: counted per reference bucket, gathered from one reference bucket, and the in-place
+// retain that keeps every term above the octave and the boundary terms the quota reaches.
+template <int W, typename T>
+struct OctaveCutArgs {
+    uint64_t* x;
+    uint64_t* z;
+    T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    uint32_t* len;
+    const uint64_t* rows;
+    int32_t ref_bits;
+    uint32_t bin;           // the boundary octave
+    uint32_t split_bucket;  // boundary terms of reference buckets below it are kept; 1 << ref_bits keeps all
+    uint32_t* counts;       // k_octave_counts: boundary terms per reference bucket
+    const uint8_t* keep;    // k_octave_retain: per column position, which of split_bucket's boundary terms stay
+    uint64_t* out_x;        // k_octave_gather: split_bucket's boundary terms, term-major [n][W], and positions
+    uint64_t* out_z;
+    uint64_t* out_pos;
+    unsigned long long* gathered;
+    uint64_t out_cap;
+};
+
+template <typename T>
+hipError_t launch_octave_hist(hipStream_t st, const OctaveHistArgs<T>& a, uint32_t blocks);
+template <int W, typename T>
+hipError_t launch_octave_counts(hipStream_t st, const OctaveCutArgs<W, T>& a, uint32_t num_buckets);
+template <int W, typename T>
+hipError_t launch_octave_gather(hipStream_t st, const OctaveCutArgs<W, T>& a, uint32_t num_buckets);
+template <int W, typename T>
+hipError_t launch_octave_retain(hipStream_t st, const OctaveCutArgs<W, T>& a, uint32_t num_buckets);
+
 }  // namespace pp

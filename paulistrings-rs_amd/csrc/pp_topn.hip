@@ -151,6 +151,136 @@ __global__ void k_mark_positions(const uint64_t* pos, uint64_t n, uint8_t* flags
     if (i < n) flags[pos[i]] = 1;
 }
 
+// ---- ApproxTopN ---------------------------------------------------------------------------------------------
+
+// octave(norm_sqr) = norm_sqr.to_bits() >> 52 (builtin.rs:276-279).  A NaN of either sign counts in octave 2047,
+// as a positive NaN does in the reference; classify_octave (:281-288) puts it below every octave.
+__device__ __forceinline__ uint32_t octave_of(uint64_t u) { return (uint32_t)((u >> 52) & 0x7FFu); }
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_octave_hist(const OctaveHistArgs<T> a) {
+    __shared__ uint32_t h[kOctaveBins];
+    for (int i = threadIdx.x; i < kOctaveBins; i += kBlock) h[i] = 0;
+    __syncthreads();
+    const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
+    const uint32_t b_hi = min(a.num_buckets, b_lo + a.buckets_per_block);
+    for (uint32_t b = b_lo; b < b_hi; ++b) {
+        const uint32_t n = a.len[b];
+        const uint64_t s = a.start[b];
+        for (uint32_t r = threadIdx.x; r < n; r += kBlock) atomicAdd(&h[octave_of(magnitude_bits<T>(a.c[s + r]))], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kOctaveBins; i += kBlock)
+        if (h[i]) atomicAdd(&a.hist[i], (unsigned long long)h[i]);
+}
+
+// The boundary side of a term: 1 above the octave, 0 at it, -1 below (or NaN).
+__device__ __forceinline__ int octave_side(uint64_t u, uint32_t bin) {
+    if (is_nan_bits(u)) return -1;
+    const uint32_t o = octave_of(u);
+    return o > bin ? 1 : (o == bin ? 0 : -1);
+}
+
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_octave_counts(const OctaveCutArgs<W, T> a) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = a.len[b];
+    const uint64_t s = a.start[b];
+    for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
+        if (octave_side(magnitude_bits<T>(a.c[s + r]), a.bin) != 0) continue;
+        const Key<W> k = load_key<W>(a.x, a.z, a.cap, s + r);
+        atomicAdd(&a.counts[bucket_of<W>(k, a.rows, a.ref_bits)], 1u);
+    }
+}
+
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_octave_gather(const OctaveCutArgs<W, T> a) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = a.len[b];
+    const uint64_t s = a.start[b];
+    for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
+        if (octave_side(magnitude_bits<T>(a.c[s + r]), a.bin) != 0) continue;
+        const Key<W> k = load_key<W>(a.x, a.z, a.cap, s + r);
+        if (bucket_of<W>(k, a.rows, a.ref_bits) != a.split_bucket) continue;
+        const uint64_t slot = atomicAdd(a.gathered, 1ull);
+        if (slot >= a.out_cap) continue;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            a.out_x[slot * W + w] = k.x[w];
+            a.out_z[slot * W + w] = k.z[w];
+        }
+        a.out_pos[slot] = s + r;
+    }
+}
+
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_octave_retain(const OctaveCutArgs<W, T> a) {
+    __shared__ uint32_t scan_scratch[kWaves];
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = a.len[b];
+    const uint64_t s = a.start[b];
+    uint32_t cur = 0;
+    for (uint32_t base = 0; base < n; base += kBlock) {
+        const uint32_t r = base + threadIdx.x;
+        Key<W> k;
+        T c = T(0);
+        bool keep = false;
+        if (r < n) {
+            k = load_key<W>(a.x, a.z, a.cap, s + r);
+            c = a.c[s + r];
+            const int side = octave_side(magnitude_bits<T>(c), a.bin);
+            if (side > 0) {
+                keep = true;
+            } else if (side == 0) {
+                const uint32_t rb = bucket_of<W>(k, a.rows, a.ref_bits);
+                keep = rb < a.split_bucket || (rb == a.split_bucket && a.keep != nullptr && a.keep[s + r] != 0);
+            }
+        }
+        bool f[1] = {keep};
+        uint32_t off[1], tot[1];
+        block_flag_scan<1>(f, off, tot, scan_scratch);  // every read of the chunk precedes its writes
+        if (keep && cur + off[0] != r) {
+            const uint64_t pos = s + cur + off[0];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                a.x[w * a.cap + pos] = k.x[w];
+                a.z[w * a.cap + pos] = k.z[w];
+            }
+            a.c[pos] = c;
+        }
+        cur += tot[0];
+    }
+    if (threadIdx.x == 0) a.len[b] = cur;
+}
+
+template <typename T>
+hipError_t launch_octave_hist(hipStream_t st, const OctaveHistArgs<T>& a, uint32_t blocks) {
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_octave_hist<T>), dim3(blocks), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int W, typename T>
+hipError_t launch_octave_counts(hipStream_t st, const OctaveCutArgs<W, T>& a, uint32_t num_buckets) {
+    if (num_buckets == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_octave_counts<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int W, typename T>
+hipError_t launch_octave_gather(hipStream_t st, const OctaveCutArgs<W, T>& a, uint32_t num_buckets) {
+    if (num_buckets == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_octave_gather<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int W, typename T>
+hipError_t launch_octave_retain(hipStream_t st, const OctaveCutArgs<W, T>& a, uint32_t num_buckets) {
+    if (num_buckets == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_octave_retain<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_topn_hist(hipStream_t st, const TopnHistArgs<T>& a, uint32_t blocks) {
     hipLaunchKernelGGL((k_topn_hist<T>), dim3(blocks), dim3(kBlock), 0, st, a);
@@ -187,9 +317,14 @@ hipError_t launch_mark_positions(hipStream_t st, const uint64_t* pos, uint64_t n
 
 template hipError_t launch_topn_hist<double>(hipStream_t, const TopnHistArgs<double>&, uint32_t);
 template hipError_t launch_topn_hist<float>(hipStream_t, const TopnHistArgs<float>&, uint32_t);
+template hipError_t launch_octave_hist<double>(hipStream_t, const OctaveHistArgs<double>&, uint32_t);
+template hipError_t launch_octave_hist<float>(hipStream_t, const OctaveHistArgs<float>&, uint32_t);
 #define PP_TOPN_INSTANTIATE(W, T)                                                                       \
     template hipError_t launch_topn_gather_ties<W, T>(hipStream_t, const TopnTieArgs<W, T>&, uint32_t); \
-    template hipError_t launch_topn_retain<W, T>(hipStream_t, const TopnRetainArgs<W, T>&, uint32_t);
+    template hipError_t launch_topn_retain<W, T>(hipStream_t, const TopnRetainArgs<W, T>&, uint32_t);   \
+    template hipError_t launch_octave_counts<W, T>(hipStream_t, const OctaveCutArgs<W, T>&, uint32_t);  \
+    template hipError_t launch_octave_gather<W, T>(hipStream_t, const OctaveCutArgs<W, T>&, uint32_t);  \
+    template hipError_t launch_octave_retain<W, T>(hipStream_t, const OctaveCutArgs<W, T>&, uint32_t);
 PP_TOPN_INSTANTIATE(1, double)
 PP_TOPN_INSTANTIATE(1, float)
 PP_TOPN_INSTANTIATE(2, double)

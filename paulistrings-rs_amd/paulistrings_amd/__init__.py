@@ -358,6 +358,10 @@ class GpuPauliSum:
     def truncate_topn(self, n: int) -> None:
         _native.check(_native.lib().pp_truncate_topn(self._h, int(n)), self.ctx.handle)
 
+    def truncate_approx_topn(self, n: int) -> None:
+        """ApproxTopN(n)'s pass: whole octaves of |c|^2 from the top, the boundary octave filled in canonical order."""
+        _native.check(_native.lib().pp_truncate_approx_topn(self._h, int(n)), self.ctx.handle)
+
     def expectation(self, state: str = "z+") -> float:
         ax, az, neg = _basis(state, self.num_qubits, self.width)
         out = ctypes.c_double()

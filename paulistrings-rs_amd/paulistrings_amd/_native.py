@@ -88,6 +88,7 @@ def lib() -> ctypes.CDLL:
     L.pp_apply_rotation.argtypes = [vp, ctypes.POINTER(Channel), ctypes.c_int, ctypes.POINTER(Policy)]
     L.pp_apply_rotation_to.argtypes = [vp, vp, ctypes.POINTER(Channel), ctypes.c_int, ctypes.POINTER(Policy)]
     L.pp_truncate_topn.argtypes = [vp, u64]
+    L.pp_truncate_approx_topn.argtypes = [vp, u64]
     L.pp_expectation_product.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
     L.pp_profile_enable.argtypes = [vp, ctypes.c_int]
     L.pp_profile_read.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64),
@@ -96,7 +97,7 @@ def lib() -> ctypes.CDLL:
     L.pp_hash_rows.restype = i32
     for name in ("pp_context_create", "pp_sync", "pp_context_stream", "pp_sum_upload", "pp_sum_clone",
                  "pp_sum_free", "pp_sum_len", "pp_sum_download", "pp_sum_bucket_bits", "pp_propagate",
-                 "pp_apply_rotation", "pp_apply_rotation_to", "pp_truncate_topn", "pp_expectation_product", "pp_profile_enable",
+                 "pp_apply_rotation", "pp_apply_rotation_to", "pp_truncate_topn", "pp_truncate_approx_topn", "pp_expectation_product", "pp_profile_enable",
                  "pp_profile_read"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L

@@ -1,4 +1,4 @@
-"""Truncation policies: ``coeff``, ``weight`` and ``topn``, composed with ``&`` and ``|``.
+"""Truncation policies: ``coeff``, ``weight``, ``topn`` and ``approx_topn``, composed with ``&`` and ``|``.
 
 Mirrors the reference's factories (crates/paulistrings-py/src/truncation.rs:11-33) and their semantics
 (crates/paulistrings/src/truncation/builtin.rs):
@@ -7,10 +7,13 @@ Mirrors the reference's factories (crates/paulistrings-py/src/truncation.rs:11-3
 * ``weight(k)`` -- WeightCutoff: keep iff the Pauli weight is at most ``k`` (builtin.rs:43-59);
 * ``topn(n)`` -- TopN: after every layer keep the ``n`` largest ``|c|``, ties at the boundary in canonical
   order (builtin.rs:61-150);
+* ``approx_topn(n)`` -- ApproxTopN: after every layer keep ``n`` terms by octave of ``|c|^2``, whole octaves from
+  the top and the boundary octave in canonical order (builtin.rs:241-468);
 * ``a & b`` -- And: both filters, both layer passes (builtin.rs:470-526);
 * ``a | b`` -- Or: either filter; an operand with a layer pass is rejected (builtin.rs:568-626).
 
-A policy lowers to the C ABI's ``pp_policy``: a postfix program of at most 15 filter nodes and one TopN pass.
+A policy lowers to the C ABI's ``pp_policy``: a postfix program of at most 15 filter nodes and one layer pass
+(TopN or ApproxTopN).
 """
 
 from __future__ import annotations
@@ -33,7 +36,7 @@ class Truncation:
         for side, name in ((self, "its first operand has"), (other, "its second operand has")):
             if side.has_pass():
                 raise ValueError("Or combines per-term filters only, but " + name +
-                                 " a layer pass (TopN); use And to add a filter to a layer pass")
+                                 " a layer pass (TopN or ApproxTopN); use And to add a filter to a layer pass")
         return Or(self, other)
 
     # -- lowering --
@@ -43,25 +46,30 @@ class Truncation:
     def topn_values(self) -> list[int]:
         return []
 
+    def approx_topn_values(self) -> list[int]:
+        return []
+
     def has_pass(self) -> bool:
-        return bool(self.topn_values())
+        return bool(self.topn_values()) or bool(self.approx_topn_values())
 
     def lower(self) -> "_native.Policy":
         nodes = self.nodes()
         if len(nodes) > 15:
             raise ValueError(f"the per-term filter has {len(nodes)} nodes; at most 15 lower to the device")
-        tops = self.topn_values()
-        if len(tops) > 1 and len(set(tops)) > 1:
-            # TopN(a) then TopN(b) keeps the min(a, b) largest, ties taken in the same order.
-            tops = [min(tops)]
+        tops, approx = self.topn_values(), self.approx_topn_values()
+        if tops and approx:
+            raise ValueError("a policy with both a TopN and an ApproxTopN layer pass does not lower to the device")
+        # TopN(a) then TopN(b) keeps the min(a, b) first by magnitude rank, ties taken in the same order; the same
+        # holds for ApproxTopN by octave rank (a prefix of a stable order is that order's prefix).
+        passes = tops or approx
         p = _native.Policy()
         p.n_nodes = len(nodes)
         for i, (op, k, eps) in enumerate(nodes):
             p.nodes[i].op = op
             p.nodes[i].k = k
             p.nodes[i].eps = eps
-        p.has_topn = 1 if tops else 0
-        p.topn = int(tops[0]) if tops else 0
+        p.has_topn = 0 if not passes else (1 if tops else 2)
+        p.topn = int(min(passes)) if passes else 0
         return p
 
 
@@ -108,6 +116,20 @@ class TopN(Truncation):
 
 
 @dataclass(frozen=True)
+class ApproxTopN(Truncation):
+    n: int
+
+    def nodes(self):
+        return []
+
+    def approx_topn_values(self):
+        return [int(self.n)]
+
+    def __repr__(self):
+        return f"approx_topn({self.n})"
+
+
+@dataclass(frozen=True)
 class And(Truncation):
     a: Truncation
     b: Truncation
@@ -122,6 +144,9 @@ class And(Truncation):
 
     def topn_values(self):
         return self.a.topn_values() + self.b.topn_values()
+
+    def approx_topn_values(self):
+        return self.a.approx_topn_values() + self.b.approx_topn_values()
 
     def __repr__(self):
         return f"({self.a!r} & {self.b!r})"
@@ -160,6 +185,14 @@ def topn(n: int) -> Truncation:
     if n < 0:
         raise ValueError(f"topn count must be non-negative, got {n}")
     return TopN(int(n))
+
+
+def approx_topn(n: int) -> Truncation:
+    """Keep ``n`` terms after every layer, ranked by octave of ``|c|^2``: whole octaves from the top while they fit,
+    the next octave filled in canonical (bucket, then key) order."""
+    if n < 0:
+        raise ValueError(f"approx_topn count must be non-negative, got {n}")
+    return ApproxTopN(int(n))
 
 
 def lower(policy: Truncation | None) -> "_native.Policy":

@@ -60,6 +60,12 @@ def test_policy_lowering_is_postfix():
         truncation.coeff(1e-3) | truncation.topn(5)
     r = (truncation.coeff(1e-2) | truncation.weight(3)).lower()
     assert [r.nodes[i].op for i in range(r.n_nodes)] == [1, 2, 4]
+    a = (truncation.coeff(1e-5) & truncation.approx_topn(1000) & truncation.approx_topn(700)).lower()
+    assert a.n_nodes == 1 and a.has_topn == 2 and a.topn == 700
+    with pytest.raises(ValueError):
+        truncation.weight(3) | truncation.approx_topn(5)
+    with pytest.raises(ValueError):
+        (truncation.topn(5) & truncation.approx_topn(5)).lower()
 
 
 def test_pauli_sum_construction_sums_duplicates_and_drops_zeros():

@@ -25,11 +25,11 @@ def announce(msg: str) -> None:
     sys.stdout.flush()
 
 
-def oracle_policy(eps=None, max_weight=None, topn=None):
-    return po.coeff_policy(eps=eps, max_weight=max_weight, topn=topn)
+def oracle_policy(eps=None, max_weight=None, topn=None, approx_topn=None):
+    return po.coeff_policy(eps=eps, max_weight=max_weight, topn=topn, approx_topn=approx_topn)
 
 
-def product_policy(eps=None, max_weight=None, topn=None):
+def product_policy(eps=None, max_weight=None, topn=None, approx_topn=None):
     parts = []
     if eps is not None:
         parts.append(truncation.coeff(eps))
@@ -37,6 +37,8 @@ def product_policy(eps=None, max_weight=None, topn=None):
         parts.append(truncation.weight(max_weight))
     if topn is not None:
         parts.append(truncation.topn(topn))
+    if approx_topn is not None:
+        parts.append(truncation.approx_topn(approx_topn))
     if not parts:
         return None
     p = parts[0]
@@ -45,12 +47,13 @@ def product_policy(eps=None, max_weight=None, topn=None):
     return p
 
 
-def run_both(num_qubits, x, z, c, circ: Circuit, direction="forward", eps=None, max_weight=None, topn=None):
+def run_both(num_qubits, x, z, c, circ: Circuit, direction="forward", eps=None, max_weight=None, topn=None,
+             approx_topn=None):
     s = po.OracleSum(num_qubits, x, z, c)
-    olens = s.propagate(circ.channels(), direction, oracle_policy(eps, max_weight, topn))
+    olens = s.propagate(circ.channels(), direction, oracle_policy(eps, max_weight, topn, approx_topn))
     ox, oz, oc, _ = s.export()
     host = PauliSum(num_qubits, x, z, c, dtype=c.dtype)
-    stats = host.propagate_with_stats(circ, product_policy(eps, max_weight, topn), direction)
+    stats = host.propagate_with_stats(circ, product_policy(eps, max_weight, topn, approx_topn), direction)
     return (ox, oz, oc, olens), (host.x_array(), host.z_array(), host.coefficients_array(), stats)
 
 
@@ -205,6 +208,43 @@ def test_topn_inside_propagation():
     x, z, c = random_sum(rng, 64, 30, 0.1, F64)
     circ = random_circuit(rng, 30, 30)
     o, p = run_both(30, x, z, c, circ, "heisenberg", eps=1e-6, topn=900)
+    assert_same(o, p)
+
+
+@pytest.mark.parametrize("case", ["normal_w2_f64", "ties_w1_f64", "normal_w2_f32", "big_w2_f64"])
+def test_approx_topn_standalone(case):
+    announce(f"approx topn {case}")
+    rng = np.random.default_rng(31)
+    if case == "ties_w1_f64":
+        nq, m = 40, 12000
+        x, z, _ = random_sum(rng, m, nq, 0.3, F64)
+        c = rng.choice([0.5, -0.5, 0.25, 0.125, -0.125, 1e-3, 0.75], size=m)
+    elif case == "big_w2_f64":  # many reference buckets: the quota runs out inside one of them
+        nq, m = 100, 300000
+        x, z, c = random_sum(rng, m, nq, 0.1, F64)
+    else:
+        nq, m = 127, 20000
+        x, z, c = random_sum(rng, m, nq, 0.1, F32 if case.endswith("f32") else F64)
+    for n in (0, 1, 777, m // 3, m - 1, m, m + 5):
+        s = po.OracleSum(nq, x, z, c)
+        s.approx_topn(n)
+        ox, oz, oc, _ = s.export()
+        g = PauliSum(nq, x, z, c, dtype=c.dtype).to_device(0)
+        g.truncate_approx_topn(n)
+        h = g.to_host()
+        g.free()
+        assert len(h.coefficients_array()) == min(n, m), (case, n)
+        assert np.array_equal(h.x_array(), ox) and np.array_equal(h.z_array(), oz), (case, n)
+        assert np.array_equal(h.coefficients_array(), oc), (case, n)
+
+
+@pytest.mark.parametrize("dtype", [F64, F32])
+def test_approx_topn_inside_propagation(dtype):
+    announce(f"approx topn inside a propagation {dtype.__name__}")
+    rng = np.random.default_rng(19)
+    x, z, c = random_sum(rng, 64, 30, 0.1, dtype)
+    circ = random_circuit(rng, 30, 30)
+    o, p = run_both(30, x, z, c, circ, "heisenberg", eps=1e-6, approx_topn=900)
     assert_same(o, p)
 
 

@@ -335,6 +335,61 @@ def test_topn_keeps_the_first_ties_in_bucket_order(n):
     assert np.array_equal(gx, ex[keep]) and np.array_equal(gz, ez[keep]) and np.array_equal(gc, ec[keep])
 
 
+def first_n_by_octave(c, n):
+    """test_support/compare.rs:33-55 with octave_rank (:63-66): stable sort descending by the exponent bits of
+    |c|^2, take n, input order.  ApproxTopN's kept set over a canonical (bucket-order) export."""
+    rank = ((c.astype(np.float64) ** 2).view(np.uint64) >> np.uint64(52)).astype(np.int64)
+    order = sorted(range(len(c)), key=lambda i: -int(rank[i]))
+    return sorted(order[:n])
+
+
+@pytest.mark.parametrize("case", ["random_w1", "tie_heavy_w2", "single_octave"])
+def test_approx_topn_keeps_the_first_n_by_octave(case):
+    # truncation/tests.rs:116-158: random sums, one magnitude group per octave, a sum inside one octave
+    rng = np.random.default_rng(0xA9C7)
+    if case == "random_w1":
+        x, z = random_terms(rng, 2000, 32, 0.3)
+        c = rng.standard_normal(2000)
+        ns = [0, 1, 37, 300, 999, 1500, 2000, 2010]
+        nq = 32
+    elif case == "tie_heavy_w2":
+        x, z = random_terms(rng, 2000, 100, 0.2)
+        c = rng.choice([1.0, -0.5, 0.25, -0.125], size=2000)
+        ns = [0, 3, 250, 700, 1200, 1900, 2500]
+        nq = 100
+    else:
+        x = np.arange(1, 5, dtype=np.uint64).reshape(-1, 1)
+        z = np.zeros((4, 1), np.uint64)
+        c = np.array([1.0, 1.125, 1.25, 1.375])
+        ns = [0, 1, 2, 3, 4]
+        nq = 32
+    for n in ns:
+        s = po.OracleSum(nq, x, z, c)
+        ex, ez, ec, _ = s.export()
+        keep = first_n_by_octave(ec, n)
+        s.approx_topn(n)
+        gx, gz, gc, _ = s.export()
+        assert len(gc) == min(n, len(ec)), (case, n)
+        assert np.array_equal(gx, ex[keep]) and np.array_equal(gz, ez[keep]) and np.array_equal(gc, ec[keep])
+
+
+def test_approx_topn_as_a_layer_pass():
+    # The pass runs after every layer (or run): one layer then ApproxTopN(n) equals the unpassed layer cut alone.
+    rng = np.random.default_rng(77)
+    x, z = random_terms(rng, 5000, 40, 0.25)
+    c = rng.standard_normal(5000)
+    ch = C.Circuit(40).rx(0.9, 3).channels()
+    a = po.OracleSum(40, x, z, c)
+    a.propagate(ch, "forward", po.make_policy(approx_topn=3000))
+    b = po.OracleSum(40, x, z, c)
+    b.propagate(ch, "forward", po.make_policy())
+    b.approx_topn(3000)
+    ax, az, ac, _ = a.export()
+    bx, bz, bc, _ = b.export()
+    assert len(ac) == 3000
+    assert np.array_equal(ax, bx) and np.array_equal(az, bz) and np.array_equal(ac, bc)
+
+
 # ---- keep programs and read-out against independent predicates ----------------------------------
 
 

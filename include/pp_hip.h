@@ -158,6 +158,43 @@ pp_status pp_profile_enable(pp_context* ctx, int enable);
 pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms, uint64_t* rows_in,
                           uint64_t* rows_out);
 
+/* ---- tuning ------------------------------------------------------------------------------------- */
+
+/* The device path's sizes, per context.  No counterpart in the reference, which fixes its device sizes
+ * (records_per_block = 4096, target_bucket_len = 1024: engine/gpu/layer/options.rs:11, pauli_sum/storage.rs:54);
+ * results do not depend on them.  Defaults come from PP_DEVICE_BUCKET_LEN, PP_JOIN_STAGE_ROWS and
+ * PP_JOIN_GROUPS_PER_CU when set. */
+typedef struct {
+    uint32_t bucket_target; /* rows per device bucket, 16..512 (default 256) */
+    uint32_t stage_rows;    /* anticommuting rows a join workgroup stages in its first launch, 64..1024 (a pair
+                               with more is redone by a second launch that stages the largest pair);
+                               0: the largest pair, no second launch (default) */
+    uint32_t groups_per_cu; /* the join's grid: 0xFFFFFFFF one workgroup per pair (default); 0 a persistent
+                               grid of as many as fit; 1..32 persistent with at most that many per CU */
+} pp_tuning;
+pp_status pp_get_tuning(pp_context* ctx, pp_tuning* out);
+pp_status pp_set_tuning(pp_context* ctx, const pp_tuning* tuning);
+/* Time candidate tunings (bucket_target 128/192/256/384 x stage_rows auto/128/256 x one-per-pair/persistent) on
+ * `sample`, each channel applied out of place from it as pp_apply_rotation_to does, keep the fastest in the
+ * context and return it (out may be NULL).  The sample is left as it was. */
+pp_status pp_autotune(pp_sum* sample, const pp_channel* channels, uint64_t n_channels, const pp_policy* policy,
+                      pp_tuning* out);
+
+/* ---- layer sequences captured into a HIP graph ---------------------------------------------- */
+
+/* No counterpart in the reference (its device engine does not capture: research/FINDINGS.md:489, :663).
+ * Between pp_graph_capture_begin and pp_graph_capture_end, the context's stream is captured instead of run:
+ * only pp_apply_rotation_to is accepted, from inputs whose counts are settled (no pending layer) and into
+ * outputs already sized by an earlier identical call; any entry point that would synchronize or allocate
+ * returns PP_ERR_UNSUPPORTED (the capture is then unusable: end it and free the graph).  pp_graph_launch
+ * replays the captured layers asynchronously on the context's stream with the captured buffers; the outputs'
+ * counts are read, as after the eager call, once the replay has run (pp_sum_len waits for it). */
+typedef struct pp_graph pp_graph;
+pp_status pp_graph_capture_begin(pp_context* ctx);
+pp_status pp_graph_capture_end(pp_context* ctx, pp_graph** out);
+pp_status pp_graph_launch(pp_graph* graph);
+pp_status pp_graph_free(pp_graph* graph);
+
 /* The reference's hash rows (Gf2Hash::new, pauli_sum/hash.rs:212-258), for checks: count*W words each. */
 int32_t pp_hash_rows(uint32_t num_qubits, uint64_t seed, uint32_t width, uint64_t* rows_x, uint64_t* rows_z);
 

@@ -30,27 +30,46 @@ constexpr uint32_t kRedrawAttempts = 4096;        // pauli_sum/hash.rs:18
 constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;  // rng.rs:5
 constexpr size_t kRunLayers = 8;                  // engine/bucketed/rotation_layer.rs:508
 constexpr int kRunDimension = 3;                  // engine/bucketed/rotation_layer.rs:511
-constexpr uint32_t kMaxPairRows = 1024;           // LDS rows of one join workgroup
+constexpr uint32_t kMaxPairRows = 1024;           // the largest pair a join's redo launch stages
+// Device counters of a sum: [0] rows, [1] longest bucket, [2] join overflow, [3] offsets total, [4] redo pairs.
+constexpr int kStatSlots = 8;
 
+// Defaults of the per-context tuning, each overridable from the environment.
 // The join's grid (PP_JOIN_GROUPS_PER_CU): one workgroup per pair by default (measured fastest on the saturated
 // layer); 1..32 selects a persistent grid of at most that many resident workgroups per CU, 0 as many as fit.
-uint32_t join_groups_per_cu() {
-    static uint32_t v = [] {
-        const char* e = std::getenv("PP_JOIN_GROUPS_PER_CU");
-        if (!e || std::strcmp(e, "pairs") == 0) return kJoinOnePairPerGroup;
-        const long n = std::strtol(e, nullptr, 10);
-        return n >= 0 && n <= 32 ? (uint32_t)n : kJoinOnePairPerGroup;
-    }();
-    return v;
+uint32_t default_groups_per_cu() {
+    const char* e = std::getenv("PP_JOIN_GROUPS_PER_CU");
+    if (!e || std::strcmp(e, "pairs") == 0) return kJoinOnePairPerGroup;
+    const long n = std::strtol(e, nullptr, 10);
+    return n >= 0 && n <= 32 ? (uint32_t)n : kJoinOnePairPerGroup;
 }
 
-uint32_t device_bucket_target() {
-    static uint32_t v = [] {
-        const char* e = std::getenv("PP_DEVICE_BUCKET_LEN");
-        long t = e ? std::strtol(e, nullptr, 10) : 0;
-        return (uint32_t)(t >= 16 && t <= 512 ? t : 256);
-    }();
-    return v;
+// Anticommuting rows one join workgroup stages in its first launch (PP_JOIN_STAGE_ROWS, 64..1024; 0 or unset: the
+// largest pair, so that no pair is redone; fewer rows raise the resident workgroups per CU, which measured no faster
+// on the saturated layer than the redo launch costs).
+uint32_t default_stage_rows() {
+    const char* e = std::getenv("PP_JOIN_STAGE_ROWS");
+    const long t = e ? std::strtol(e, nullptr, 10) : 0;
+    return (uint32_t)(t >= 64 && t <= 1024 ? t : 0);
+}
+
+// Rows per device bucket (PP_DEVICE_BUCKET_LEN, 16..512; default 256).
+uint32_t default_bucket_target() {
+    const char* e = std::getenv("PP_DEVICE_BUCKET_LEN");
+    const long t = e ? std::strtol(e, nullptr, 10) : 0;
+    return (uint32_t)(t >= 16 && t <= 512 ? t : 256);
+}
+
+bool valid_tuning(const pp_tuning& t) {
+    return t.bucket_target >= 16 && t.bucket_target <= 512 && (t.stage_rows == 0 || (t.stage_rows >= 64 && t.stage_rows <= 1024)) &&
+           (t.groups_per_cu <= 32 || t.groups_per_cu == kJoinOnePairPerGroup);
+}
+
+// The staging rows of a join whose largest pair is lcap rows: the tuning's (or lcap), a multiple of 64, at most lcap.
+uint32_t join_stage_rows(uint32_t tuned, uint32_t lcap) {
+    uint32_t rows = tuned ? tuned : lcap;
+    rows = (rows + 63) & ~63u;
+    return std::min(rows, lcap);
 }
 
 // ---- the reference's hash rows and bucket-count policy ---------------------------------------------------------
@@ -161,6 +180,8 @@ struct pp_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string error;
+    bool capturing = false;  // between pp_graph_capture_begin and _end: no allocation, no synchronization
+    pp_tuning tuning{};      // sizes of the device path (pp_set_tuning, pp_autotune)
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     uint64_t prof_rows_in = 0;
@@ -203,6 +224,8 @@ struct pp_sum {
     unsigned long long* h_stats = nullptr;  // pinned
     pp::TopnState* d_topn = nullptr;
     unsigned long long* d_hist = nullptr;
+    uint32_t* d_redo = nullptr;  // pairs a join could not stage, for its redo launch
+    uint64_t redo_cap = 0;
     size_t coeff_bytes() const { return kind == PP_COEFF_F64 ? 8 : 4; }
 };
 
@@ -213,6 +236,14 @@ pp_status fail(pp_context* ctx, pp_status s, const std::string& msg) {
     return s;
 }
 
+// Entry points that synchronize or allocate refuse to run inside a graph capture.
+#define PP_NOT_CAPTURING(ctx, what)                                                                     \
+    do {                                                                                               \
+        if ((ctx) && (ctx)->capturing)                                                                 \
+            return fail((ctx), PP_ERR_UNSUPPORTED,                                                     \
+                        std::string(what) + " is not capturable: close the capture first");           \
+    } while (0)
+
 #define PP_HIP(ctx, call)                                                                              \
     do {                                                                                               \
         hipError_t e_ = (call);                                                                        \
@@ -222,6 +253,10 @@ pp_status fail(pp_context* ctx, pp_status s, const std::string& msg) {
 
 pp_status dev_alloc(pp_context* ctx, void** p, size_t bytes) {
     *p = nullptr;
+    if (ctx->capturing)
+        return fail(ctx, PP_ERR_UNSUPPORTED,
+                    "an allocation during graph capture: run the same calls once before capturing so that every "
+                    "buffer has its size");
     if (bytes == 0) bytes = 64;
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
@@ -296,14 +331,18 @@ uint32_t bucket_of_host(const pp_sum* s, const uint64_t* gx, const uint64_t* gz,
 
 pp_status read_stats(pp_sum* s) {
     pp_context* ctx = s->ctx;
-    PP_HIP(ctx, hipMemcpyAsync(s->h_stats, s->d_stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+    if (ctx->capturing)
+        return fail(ctx, PP_ERR_UNSUPPORTED,
+                    "a host read-back during graph capture: only pp_apply_rotation_to from a settled, non-pending "
+                    "input is capturable");
+    PP_HIP(ctx, hipMemcpyAsync(s->h_stats, s->d_stats, kStatSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                ctx->stream));
     PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PP_OK;
 }
 
 pp_status clear_stats(pp_sum* s) {
-    PP_HIP(s->ctx, hipMemsetAsync(s->d_stats, 0, 4 * sizeof(unsigned long long), s->ctx->stream));
+    PP_HIP(s->ctx, launch_fill_u64(s->ctx->stream, s->d_stats, kStatSlots, 0ull));
     return PP_OK;
 }
 
@@ -413,10 +452,10 @@ pp_status coarsen_to(pp_sum* s, int bits) {
     return PP_OK;
 }
 
-// The device's own bucket count: about device_bucket_target() rows per bucket, with a factor-8 hysteresis, and
+// The device's own bucket count: about the context's bucket_target rows per bucket, with a factor-8 hysteresis, and
 // fine enough that the largest pair fits one workgroup's LDS.
 pp_status settle_device_bits(pp_sum* s) {
-    const uint64_t target = device_bucket_target();
+    const uint64_t target = s->ctx->tuning.bucket_target;
     int want = 0;
     while (want < s->max_bits && s->len > target * (1ull << want)) ++want;
     pp_status st;
@@ -532,14 +571,31 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
     if ((st = clear_stats(dst)) != PP_OK) return st;
     const uint32_t delta = bucket_of_host(src, ch.gx, ch.gz, src->device_bits);
     if ((st = offsets(dst, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start)) != PP_OK) return st;
+    // lcap bounds the largest pair; the first launch stages scap anticommuting rows per pair, and the pairs with
+    // more (rare: a generator anticommutes with a small fraction of a random bucket) go to a redo launch that
+    // stages lcap.  LDS per workgroup, and so the resident workgroups per CU, follow scap.
     uint32_t lcap = delta ? 2u * src->max_bucket : src->max_bucket;
     lcap = std::max<uint32_t>(64, (lcap + 63) & ~63u);
     if (lcap > kMaxPairRows) lcap = kMaxPairRows;
-    uint32_t tcap_bits = 1;
-    while ((1u << tcap_bits) < 2u * lcap) ++tcap_bits;
+    const uint32_t scap = join_stage_rows(ctx->tuning.stage_rows, lcap);
+    auto table_bits_for = [](uint32_t rows) {
+        uint32_t b = 1;
+        while ((1u << b) < 2u * rows) ++b;
+        return b;
+    };
     const uint32_t cosets = delta ? B / 2 : B;
+    // sized for B whatever this channel's cosets, so that no later layer on the same bucketing reallocates it (a
+    // captured graph holds the pointer)
+    if (dst->redo_cap < B) {
+        if (dst->d_redo) (void)hipFree(dst->d_redo);
+        dst->d_redo = nullptr;
+        dst->redo_cap = 0;
+        if ((st = dev_alloc(ctx, (void**)&dst->d_redo, (size_t)B * 4)) != PP_OK) return st;
+        dst->redo_cap = B;
+    }
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (ctx->profiling) {
+    const bool timed = ctx->profiling && !ctx->capturing;
+    if (timed) {
         PP_HIP(ctx, hipEventCreate(&ev0));
         PP_HIP(ctx, hipEventCreate(&ev1));
         PP_HIP(ctx, hipEventRecord(ev0, ctx->stream));
@@ -562,8 +618,8 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         a.olen = dst->bk[out].len;
         a.delta = delta;
         a.hb = delta ? 31u - (uint32_t)__builtin_clz(delta) : 0u;
-        a.lcap = lcap;
-        a.tcap_bits = tcap_bits;
+        a.scap = scap;
+        a.tcap_bits = table_bits_for(scap);
         a.num_cosets = cosets;
         uint32_t gphase = 0;
         for (uint32_t w = 0; w < W; ++w) {
@@ -577,16 +633,22 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         a.clifford = p.clifford ? 1 : 0;
         a.keep = keep;
         a.stats = dst->d_stats;
-        return launch_rotation_join<W, T>(ctx->stream, a, join_groups_per_cu());
+        a.redo = dst->d_redo;
+        a.redo_count = dst->d_stats + 4;
+        hipError_t le = launch_rotation_join<W, T>(ctx->stream, a, ctx->tuning.groups_per_cu);
+        if (le != hipSuccess || scap >= lcap) return le;  // a first launch staging lcap never queues a pair
+        a.redo_mode = 1;
+        a.scap = lcap;
+        a.tcap_bits = table_bits_for(lcap);
+        return launch_rotation_join<W, T>(ctx->stream, a, 0);
     });
     PP_HIP(ctx, e);
-    if (ctx->profiling) {
+    if (timed) {
         PP_HIP(ctx, hipEventRecord(ev1, ctx->stream));
         ctx->events.push_back({ev0, ev1});
         ctx->prof_rows_in += src->len;
     }
-    PP_HIP(ctx, launch_reduce_lens(ctx->stream, dst->bk[out].len, B, dst->d_stats,
-                                   ctx->profiling ? ctx->d_prof : nullptr));
+    PP_HIP(ctx, launch_reduce_lens(ctx->stream, dst->bk[out].len, B, dst->d_stats, timed ? ctx->d_prof : nullptr));
     return PP_OK;
 }
 
@@ -622,7 +684,7 @@ pp_status device_layer(pp_sum* s, const pp_channel& ch, const Prepared& p, const
     const int out = 1 - s->cur;
     if ((st = launch_join(s, s, out, ch, p, keep)) != PP_OK) return st;
     if ((st = read_stats(s)) != PP_OK) return st;
-    if (s->h_stats[2]) return fail(ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded its LDS rows");
+    if (s->h_stats[2]) return fail(ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
     s->len = s->h_stats[0];
     s->max_bucket = (uint32_t)s->h_stats[1];
     s->cur = out;
@@ -635,7 +697,7 @@ pp_status refresh(pp_sum* s) {
     pp_status st;
     if ((st = read_stats(s)) != PP_OK) return st;
     s->stale = false;
-    if (s->h_stats[2]) return fail(s->ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded its LDS rows");
+    if (s->h_stats[2]) return fail(s->ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
     s->len = s->h_stats[0];
     s->max_bucket = (uint32_t)s->h_stats[1];
     return PP_OK;
@@ -669,6 +731,9 @@ pp_status pp_context_create(int device, pp_context** out) {
     auto* ctx = new (std::nothrow) pp_context();
     if (!ctx) return PP_ERR_OUT_OF_MEMORY;
     ctx->device = device;
+    ctx->tuning.bucket_target = default_bucket_target();
+    ctx->tuning.stage_rows = default_stage_rows();
+    ctx->tuning.groups_per_cu = default_groups_per_cu();
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return PP_ERR_DRIVER;
@@ -693,6 +758,7 @@ const char* pp_last_error(const pp_context* ctx) { return ctx ? ctx->error.c_str
 
 pp_status pp_sync(pp_context* ctx) {
     if (!ctx) return PP_ERR_INVALID_ARGUMENT;
+    PP_NOT_CAPTURING(ctx, "pp_sync");
     PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PP_OK;
 }
@@ -711,6 +777,7 @@ int32_t pp_hash_rows(uint32_t num_qubits, uint64_t seed, uint32_t width, uint64_
 pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp_coeff_kind kind, uint64_t n,
                         const uint64_t* x, const uint64_t* z, const void* coeff, uint64_t hash_seed,
                         pp_sum** out) {
+    PP_NOT_CAPTURING(ctx, "pp_sum_upload");
     if (!ctx || !out) return PP_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     if (width != 1 && width != 2) return fail(ctx, PP_ERR_UNSUPPORTED, "width must be 1 or 2");
@@ -739,8 +806,8 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
     };
     pp_status st;
     if ((st = dev_alloc(ctx, (void**)&s->d_rows, s->rows.size() * 8)) != PP_OK) return bail(st);
-    if ((st = dev_alloc(ctx, (void**)&s->d_stats, 4 * sizeof(unsigned long long))) != PP_OK) return bail(st);
-    if (hipHostMalloc((void**)&s->h_stats, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+    if ((st = dev_alloc(ctx, (void**)&s->d_stats, kStatSlots * sizeof(unsigned long long))) != PP_OK) return bail(st);
+    if (hipHostMalloc((void**)&s->h_stats, kStatSlots * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return bail(fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned host allocation"));
     if (hipMemcpy(s->d_rows, s->rows.data(), s->rows.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(ctx, PP_ERR_DRIVER, "hash rows upload"));
@@ -757,7 +824,7 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
     const uint64_t m = keep_idx.size();
     s->len = m;
     s->reference_bits = std::min(desired_bits(m, 1024, 128), s->max_bits);
-    const uint64_t target = device_bucket_target();
+    const uint64_t target = ctx->tuning.bucket_target;
     int dbits = 0;
     while (dbits < s->max_bits && m > target * (1ull << dbits)) ++dbits;
     s->device_bits = dbits;
@@ -806,6 +873,7 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
 }
 
 pp_status pp_sum_free(pp_sum* s) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_sum_free");
     if (!s) return PP_OK;
     if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
     for (int i = 0; i < 2; ++i) {
@@ -816,6 +884,7 @@ pp_status pp_sum_free(pp_sum* s) {
     if (s->d_stats) (void)hipFree(s->d_stats);
     if (s->h_stats) (void)hipHostFree(s->h_stats);
     if (s->d_topn) (void)hipFree(s->d_topn);
+    if (s->d_redo) (void)hipFree(s->d_redo);
     if (s->d_hist) (void)hipFree(s->d_hist);
     if (s->d_tiles) (void)hipFree(s->d_tiles);
     delete s;
@@ -823,6 +892,7 @@ pp_status pp_sum_free(pp_sum* s) {
 }
 
 pp_status pp_sum_clone(const pp_sum* src_c, pp_sum** out) {
+    if (src_c) PP_NOT_CAPTURING(src_c->ctx, "pp_sum_clone");
     if (!src_c || !out) return PP_ERR_INVALID_ARGUMENT;
     pp_sum* src = const_cast<pp_sum*>(src_c);
     pp_context* ctx = src->ctx;
@@ -847,8 +917,8 @@ pp_status pp_sum_clone(const pp_sum* src_c, pp_sum** out) {
     };
     pp_status st;
     if ((st = dev_alloc(ctx, (void**)&s->d_rows, s->rows.size() * 8)) != PP_OK) return bail(st);
-    if ((st = dev_alloc(ctx, (void**)&s->d_stats, 4 * sizeof(unsigned long long))) != PP_OK) return bail(st);
-    if (hipHostMalloc((void**)&s->h_stats, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+    if ((st = dev_alloc(ctx, (void**)&s->d_stats, kStatSlots * sizeof(unsigned long long))) != PP_OK) return bail(st);
+    if (hipHostMalloc((void**)&s->h_stats, kStatSlots * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return bail(fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned host allocation"));
     const Columns& sc = src->col[src->cur];
     const BucketArrays& sb = src->bk[src->cur];
@@ -877,6 +947,7 @@ pp_status pp_sum_clone(const pp_sum* src_c, pp_sum** out) {
 }
 
 pp_status pp_sum_len(pp_sum* s, uint64_t* len) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_sum_len");
     if (!s || !len) return PP_ERR_INVALID_ARGUMENT;
     pp_status st = refresh(s);
     if (st != PP_OK) return st;
@@ -892,6 +963,7 @@ pp_status pp_sum_bucket_bits(pp_sum* s, int32_t* reference_bits, int32_t* device
 }
 
 pp_status pp_sum_download(pp_sum* s, uint64_t* x, uint64_t* z, void* coeff) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_sum_download");
     if (!s) return PP_ERR_INVALID_ARGUMENT;
     pp_context* ctx = s->ctx;
     {
@@ -983,6 +1055,7 @@ pp_status pp_sum_download(pp_sum* s, uint64_t* x, uint64_t* z, void* coeff) {
 }
 
 pp_status pp_apply_rotation(pp_sum* s, const pp_channel* ch, int adjoint, const pp_policy* policy) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_apply_rotation");
     if (!s || !ch) return PP_ERR_INVALID_ARGUMENT;
     if (!valid_policy(policy)) return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
     pp_status st = refresh(s);
@@ -999,6 +1072,8 @@ pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* ch, i
         return fail(ctx, PP_ERR_INVALID_ARGUMENT, "pp_apply_rotation_to: the sums differ in shape or context");
     if (!valid_policy(policy)) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
     pp_status st;
+    if (ctx->capturing && src->stale)
+        return fail(ctx, PP_ERR_UNSUPPORTED, "pp_apply_rotation_to: the input's counts are pending during capture");
     if ((st = refresh(src)) != PP_OK) return st;
     if ((st = settle_device_bits(src)) != PP_OK) return st;
     const Prepared p = prepare(src, *ch, adjoint != 0);
@@ -1014,6 +1089,7 @@ pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* ch, i
 
 pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channels, pp_direction direction,
                        const pp_policy* policy, const pp_options* options, uint64_t* layer_len) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_propagate");
     if (!s || (n_channels && !channels)) return PP_ERR_INVALID_ARGUMENT;
     {
         pp_status rs = refresh(s);
@@ -1099,6 +1175,7 @@ pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channel
 }
 
 pp_status pp_truncate_topn(pp_sum* s, uint64_t n) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_truncate_topn");
     if (!s) return PP_ERR_INVALID_ARGUMENT;
     pp_status rs = refresh(s);
     if (rs != PP_OK) return rs;
@@ -1107,6 +1184,7 @@ pp_status pp_truncate_topn(pp_sum* s, uint64_t n) {
 }
 
 pp_status pp_truncate_approx_topn(pp_sum* s, uint64_t n) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_truncate_approx_topn");
     if (!s) return PP_ERR_INVALID_ARGUMENT;
     pp_status rs = refresh(s);
     if (rs != PP_OK) return rs;
@@ -1116,6 +1194,7 @@ pp_status pp_truncate_approx_topn(pp_sum* s, uint64_t n) {
 
 pp_status pp_expectation_product(pp_sum* s, const uint64_t* axis_x, const uint64_t* axis_z, const uint64_t* negative,
                                  double* out) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_expectation_product");
     if (!s || !axis_x || !axis_z || !negative || !out) return PP_ERR_INVALID_ARGUMENT;
     pp_context* ctx = s->ctx;
     {
@@ -1160,6 +1239,7 @@ pp_status pp_expectation_product(pp_sum* s, const uint64_t* axis_x, const uint64
 }
 
 pp_status pp_profile_enable(pp_context* ctx, int enable) {
+    PP_NOT_CAPTURING(ctx, "pp_profile_enable");
     if (!ctx) return PP_ERR_INVALID_ARGUMENT;
     PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (auto& e : ctx->events) {
@@ -1180,6 +1260,7 @@ pp_status pp_profile_enable(pp_context* ctx, int enable) {
 
 pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms, uint64_t* rows_in,
                           uint64_t* rows_out) {
+    PP_NOT_CAPTURING(ctx, "pp_profile_read");
     if (!ctx) return PP_ERR_INVALID_ARGUMENT;
     PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     double ms = 0.0;
@@ -1198,6 +1279,150 @@ pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms,
 }
 
 }  // extern "C"
+
+// ---- tuning -------------------------------------------------------------------------------------------------------
+
+pp_status pp_get_tuning(pp_context* ctx, pp_tuning* out) {
+    if (!ctx || !out) return PP_ERR_INVALID_ARGUMENT;
+    *out = ctx->tuning;
+    return PP_OK;
+}
+
+pp_status pp_set_tuning(pp_context* ctx, const pp_tuning* t) {
+    if (!ctx || !t) return PP_ERR_INVALID_ARGUMENT;
+    PP_NOT_CAPTURING(ctx, "pp_set_tuning");
+    if (!valid_tuning(*t)) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "tuning out of range");
+    ctx->tuning = *t;
+    return PP_OK;
+}
+
+pp_status pp_autotune(pp_sum* sample, const pp_channel* channels, uint64_t n_channels, const pp_policy* policy,
+                      pp_tuning* out) {
+    if (!sample || (!channels && n_channels)) return PP_ERR_INVALID_ARGUMENT;
+    pp_context* ctx = sample->ctx;
+    PP_NOT_CAPTURING(ctx, "pp_autotune");
+    if (!valid_policy(policy)) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
+    pp_status st = refresh(sample);
+    if (st != PP_OK) return st;
+    // the channels the out-of-place layer takes (a key-preserving one runs in place)
+    std::vector<pp_channel> chans;
+    for (uint64_t i = 0; i < n_channels; ++i)
+        if (!prepare(sample, channels[i], false).key_preserving) chans.push_back(channels[i]);
+    const pp_tuning saved = ctx->tuning;
+    const bool profiling = ctx->profiling;
+    ctx->profiling = false;  // the trials' joins stay out of the measurement counters
+    pp_tuning best = saved;
+    double best_ms = -1.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PP_HIP(ctx, hipEventCreate(&e0));
+    PP_HIP(ctx, hipEventCreate(&e1));
+    auto trial = [&](const pp_tuning& t, double* ms) -> pp_status {
+        ctx->tuning = t;
+        pp_sum *a = nullptr, *b = nullptr;
+        pp_status rs = pp_sum_clone(sample, &a);
+        if (rs == PP_OK) rs = settle_device_bits(a);  // the candidate's bucket count
+        if (rs == PP_OK) rs = pp_sum_clone(a, &b);
+        for (int pass = 0; pass < 2 && rs == PP_OK; ++pass) {  // the first pass sizes b's buffers
+            if (pass == 1 && hipEventRecord(e0, ctx->stream) != hipSuccess) rs = PP_ERR_DRIVER;
+            for (const pp_channel& ch : chans)
+                if (rs == PP_OK) rs = pp_apply_rotation_to(a, b, &ch, 0, policy);
+            if (rs == PP_OK && pass == 1 && hipEventRecord(e1, ctx->stream) != hipSuccess) rs = PP_ERR_DRIVER;
+            if (rs == PP_OK) rs = refresh(b);
+        }
+        float f = 0.f;
+        if (rs == PP_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&f, e0, e1) != hipSuccess))
+            rs = fail(ctx, PP_ERR_DRIVER, "autotune timing");
+        *ms = f;
+        if (b) pp_sum_free(b);
+        if (a) pp_sum_free(a);
+        return rs;
+    };
+    const uint32_t targets[] = {128, 192, 256, 384};
+    const uint32_t stages[] = {0, 128, 256};
+    const uint32_t grids[] = {kJoinOnePairPerGroup, 0};
+    for (uint32_t bt : targets)
+        for (uint32_t sr : stages)
+            for (uint32_t gp : grids) {
+                pp_tuning t{bt, sr, gp};
+                double ms = 0;
+                if ((st = trial(t, &ms)) != PP_OK) break;
+                if (best_ms < 0 || ms < best_ms) {
+                    best_ms = ms;
+                    best = t;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->profiling = profiling;
+    ctx->tuning = st == PP_OK ? best : saved;
+    if (st != PP_OK) return st;
+    if (out) *out = best;
+    return PP_OK;
+}
+
+// ---- layer sequences captured into a HIP graph ----------------------------------------------------------------
+
+struct pp_graph {
+    pp_context* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+pp_status pp_graph_capture_begin(pp_context* ctx) {
+    if (!ctx) return PP_ERR_INVALID_ARGUMENT;
+    if (ctx->capturing) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "a capture is already open on this context");
+    PP_HIP(ctx, hipSetDevice(ctx->device));
+    PP_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    ctx->capturing = true;
+    return PP_OK;
+}
+
+pp_status pp_graph_capture_end(pp_context* ctx, pp_graph** out) {
+    if (!ctx || !out) return PP_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!ctx->capturing) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "no capture is open on this context");
+    ctx->capturing = false;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (e != hipSuccess || !graph) {
+        (void)hipGetLastError();
+        return fail(ctx, PP_ERR_DRIVER, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(ctx, PP_ERR_DRIVER, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    auto* g = new (std::nothrow) pp_graph();
+    if (!g) {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        return fail(ctx, PP_ERR_OUT_OF_MEMORY, "host allocation");
+    }
+    g->ctx = ctx;
+    g->graph = graph;
+    g->exec = exec;
+    *out = g;
+    return PP_OK;
+}
+
+pp_status pp_graph_launch(pp_graph* g) {
+    if (g) PP_NOT_CAPTURING(g->ctx, "pp_graph_launch");
+    if (!g) return PP_ERR_INVALID_ARGUMENT;
+    PP_HIP(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream));
+    return PP_OK;
+}
+
+pp_status pp_graph_free(pp_graph* g) {
+    if (g) PP_NOT_CAPTURING(g->ctx, "pp_graph_free");
+    if (!g) return PP_OK;
+    (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return PP_OK;
+}
 
 // The TopN pass lives in pp_topn.hip; it needs the sum's internals.
 #include "pp_topn_impl.inc"

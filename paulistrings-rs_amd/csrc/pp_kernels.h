@@ -27,8 +27,8 @@ struct JoinArgs {
     uint32_t* olen;
     uint32_t delta;      // h(g) under the device bucket bits; 0 makes every coset a single bucket
     uint32_t hb;         // highest set bit of delta
-    uint32_t lcap;       // LDS rows: at least the largest pair
-    uint32_t tcap_bits;  // LDS table slots: 1 << tcap_bits >= 2 * lcap
+    uint32_t scap;       // staged (anticommuting) rows one workgroup holds in LDS; a pair with more is redone
+    uint32_t tcap_bits;  // LDS table slots: 1 << tcap_bits >= 2 * scap
     uint32_t num_cosets; // pairs {b, b ^ delta} (every bucket on its own when delta = 0)
     uint32_t gphase;     // sum over words of |gx & gz|, the generator's own term of the product phase
     uint64_t gx[W];
@@ -37,7 +37,10 @@ struct JoinArgs {
     T sin_t;
     int32_t clifford;
     KeepProgram keep;
-    unsigned long long* stats;  // [2] |= LDS overflow (never expected: the host sizes lcap for the largest pair)
+    unsigned long long* stats;  // [2] |= overflow in the redo launch (never expected: it stages the largest pair)
+    uint32_t* redo;                  // pairs the first launch could not stage, for the redo launch
+    unsigned long long* redo_count;  // their number
+    uint32_t redo_mode;              // 0: every pair; 1: the queued pairs, staging up to scap = the largest pair
 };
 
 template <int W, typename T>
@@ -132,12 +135,14 @@ uint32_t offsets_tiles(uint32_t out_n);
 hipError_t launch_bucket_offsets(hipStream_t st, const uint32_t* len, uint32_t num_buckets, uint32_t mode,
                                  uint32_t delta, uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start,
                                  unsigned long long* total, uint64_t* tile_sums);
+// p[0 .. n) = v, as a kernel (capturable).
+hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, uint32_t n, unsigned long long v);
 // out[0] += total rows, out[1] = max(out[1], longest bucket); prof[0] += total rows when prof is given.
 hipError_t launch_reduce_lens(hipStream_t st, const uint32_t* len, uint32_t num_buckets, unsigned long long* out,
                               unsigned long long* prof);
 
 template <int W, typename T>
-size_t join_lds_bytes(uint32_t lcap, uint32_t tcap_bits);
+size_t join_lds_bytes(uint32_t scap, uint32_t tcap_bits);
 // groups_per_cu caps the resident workgroups per CU of the persistent grid (0: as many as fit;
 // kJoinOnePairPerGroup: a workgroup per pair, no persistence).
 constexpr uint32_t kJoinOnePairPerGroup = 0xFFFFFFFFu;

@@ -259,13 +259,13 @@ __device__ __forceinline__ uint32_t table_bits(uint32_t n) {
 // Phase B over staged rows lb .. lb + kBlock - 1 of a pair (see k_rotation_join).
 template <int W, typename T>
 struct Staged {
-    uint64_t* x;  // [W][lcap]
-    uint64_t* z;  // [W][lcap]
+    uint64_t* x;  // [W][stride]
+    uint64_t* z;  // [W][stride]
     T* c;
     uint32_t* fold;
     uint8_t* side;  // 0: the row came from b0, 1: from b1
     uint32_t* table;
-    uint32_t lcap;
+    uint32_t stride;  // staged rows held (the word planes' stride)
 };
 
 template <int W, typename T>
@@ -274,7 +274,7 @@ __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const 
                                                  uint32_t lb, const Staged<W, T>& st, uint32_t& cur0,
                                                  uint32_t& cur1, uint32_t* scan_scratch) {
     const bool pair = a.delta != 0;
-    const uint32_t lcap = st.lcap;
+    const uint32_t stride = st.stride;
     const uint64_t* Lx = st.x;
     const uint64_t* Lz = st.z;
     const T* Lc = st.c;
@@ -291,8 +291,8 @@ __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const 
     if (valid) {
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            k.x[w] = Lx[w * lcap + i];
-            k.z[w] = Lz[w * lcap + i];
+            k.x[w] = Lx[w * stride + i];
+            k.z[w] = Lz[w * stride + i];
         }
         const T c = Lc[i];
         j = Lm[i];
@@ -309,7 +309,7 @@ __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const 
                 bool eq = true;
 #pragma unroll
                 for (int w = 0; w < W; ++w)
-                    eq = eq && (Lx[w * lcap + idx] == q.x[w]) && (Lz[w * lcap + idx] == q.z[w]);
+                    eq = eq && (Lx[w * stride + idx] == q.x[w]) && (Lz[w * stride + idx] == q.z[w]);
                 if (eq) {
                     found = (int)idx;
                     break;
@@ -347,22 +347,29 @@ __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const 
     cur1 += tot[1] + tot[3];
 }
 
-template <int W, typename T>
+// REDO: the launch over the queued pairs (a compile-time switch, so that the first launch's pair index needs no
+// load: a lookup there, merged with the prefetch's path, would make the compiler wait for the prefetch).
+template <int W, typename T, bool REDO>
 __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t scan_scratch[4 * kWaves];
-    const uint32_t lcap = a.lcap;
-    uint64_t* const Lx = reinterpret_cast<uint64_t*>(lds);   // [W][lcap]
-    uint64_t* const Lz = Lx + (size_t)W * lcap;              // [W][lcap]
-    T* const Lc = reinterpret_cast<T*>(Lz + (size_t)W * lcap);
-    uint32_t* const Lf = reinterpret_cast<uint32_t*>(Lc + lcap);  // key_fold of each staged row
-    uint32_t* const table = Lf + lcap;
+    const uint32_t scap = a.scap;  // staged rows this launch holds
+    uint64_t* const Lx = reinterpret_cast<uint64_t*>(lds);   // [W][scap]
+    uint64_t* const Lz = Lx + (size_t)W * scap;              // [W][scap]
+    T* const Lc = reinterpret_cast<T*>(Lz + (size_t)W * scap);
+    uint32_t* const Lf = reinterpret_cast<uint32_t*>(Lc + scap);  // key_fold of each staged row
+    uint32_t* const table = Lf + scap;
     uint8_t* const Lm = reinterpret_cast<uint8_t*>(table + (1u << a.tcap_bits));
-    const Staged<W, T> st{Lx, Lz, Lc, Lf, Lm, table, lcap};
+    const Staged<W, T> st{Lx, Lz, Lc, Lf, Lm, table, scap};
 
-    const uint32_t G = gridDim.x, last = a.num_cosets - 1u;
-    uint32_t p = blockIdx.x;
-    if (p > last) return;  // the host launches at most num_cosets workgroups
+    // Work item t is pair t, or in the redo launch the t-th pair the first launch queued.
+    constexpr bool redo = REDO;
+    const uint32_t count = redo ? (uint32_t)*a.redo_count : a.num_cosets;
+    const uint32_t G = gridDim.x;
+    uint32_t t = blockIdx.x;
+    if (t >= count) return;  // uniform
+    const uint32_t last = count - 1u;
+    auto pair_at = [&](uint32_t i) { return redo ? a.redo[i] : i; };
     const bool pair = a.delta != 0;
     Key<W> g;
 #pragma unroll
@@ -373,31 +380,30 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
     const uint32_t supp = support_words<W>(g);
     const uint32_t fold_g = key_fold<W>(g);
 
+    uint32_t p = pair_at(t);
     PairMeta m = read_meta(a, p, load_meta(a, p));
     ChunkRow<W, T> cb;
     load_chunk_row(a, m, threadIdx.x, cb);
-    MetaWords mw = load_meta(a, min(p + G, last));
+    MetaWords mw = load_meta(a, pair_at(min(t + G, last)));
     // Land the first chunk here, so that inside the loop the current chunk is always one copied from the
     // previous iteration's prefetch (which the copy waited for) and its first use waits on nothing younger.
     __builtin_amdgcn_s_waitcnt(0);
     uint32_t base = 0, cur0 = 0, cur1 = 0, nl = 0;
-    // clamped to the allocation: a pair past lcap is refused below, before any insert
-    uint32_t tbits = min(table_bits(m.n0 + m.n1), a.tcap_bits);
+    bool overflow = false;  // more anticommuting rows than scap: the pair is left to the redo launch
+    uint32_t tbits = min(table_bits(min(m.n0 + m.n1, scap)), a.tcap_bits);
     for (uint32_t i = threadIdx.x; i < (1u << tbits); i += kBlock) table[i] = 0u;
     for (;;) {
         const uint32_t n = m.n0 + m.n1;
-        if (n > lcap) {  // the host sizes lcap for the largest pair; never taken, guards the LDS bounds
-            if (threadIdx.x == 0) atomicOr(&a.stats[2], 1ull);
-            return;  // uniform: every wave of the workgroup leaves here
-        }
         // ---- the next chunk: this pair's, or the first of the next pair ----
         const bool last_chunk = base + kBlock >= n;
         bool have_next = true;
         PairMeta mn = m;
         uint32_t nbase = base + kBlock;
+        uint32_t pn = p;
         if (last_chunk) {
-            if (p + G <= last) {
-                mn = read_meta(a, p + G, mw);
+            if (t + G <= last) {
+                pn = pair_at(t + G);
+                mn = read_meta(a, pn, mw);
                 nbase = 0;
             } else {
                 have_next = false;
@@ -405,10 +411,10 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
         }
         ChunkRow<W, T> nb;
         if (have_next) load_chunk_row(a, mn, nbase + threadIdx.x, nb);
-        if (last_chunk && have_next) mw = load_meta(a, min(p + 2u * G, last));
+        if (last_chunk && have_next) mw = load_meta(a, pair_at(min(t + 2u * G, last)));
 
         // ---- phase A over the current chunk ----
-        {
+        if (!overflow) {  // uniform
             const uint32_t r = base + threadIdx.x;
             const bool valid = r < n;
             const uint32_t j = r >= m.n0 ? 1u : 0u;
@@ -420,17 +426,18 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
             bool f[3] = {keep && j == 0, keep && j == 1, anti};
             uint32_t off[3], tot[3];
             block_flag_scan<3>(f, off, tot, scan_scratch);
-            if (keep) {  // one store sequence for either bucket
+            overflow = nl + tot[2] > scap;
+            if (keep) {  // one store sequence for either bucket (a redo rewrites the same positions)
                 const uint64_t pos = j == 0 ? m.o0 + cur0 + off[0] : m.o1 + cur1 + off[1];
                 store_key<W>(a.ox, a.oz, a.ocap, pos, cb.k);
                 store_coeff<T>(a.oc, pos, cb.c);
             }
-            if (anti) {
+            if (anti && !overflow) {
                 const uint32_t li = nl + off[2];
 #pragma unroll
                 for (int w = 0; w < W; ++w) {
-                    Lx[w * lcap + li] = cb.k.x[w];
-                    Lz[w * lcap + li] = cb.k.z[w];
+                    Lx[w * scap + li] = cb.k.x[w];
+                    Lz[w * scap + li] = cb.k.z[w];
                 }
                 Lc[li] = cb.c;
                 Lm[li] = (uint8_t)j;
@@ -449,31 +456,46 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
         }
 
         if (last_chunk) {
-            // ---- phase B over the pair's staged rows ----
-            // One pass in the common case nl <= kBlock: a loop holding stores and no loads would make the
-            // compiler wait for the prefetch before entering it.
-            if (nl != 0) {
-                __syncthreads();  // the staged rows and the table are complete
-                if (nl <= kBlock) {
-                    join_staged_pass<W, T>(a, m, g, supp, fold_g, tbits, nl, 0, st, cur0, cur1, scan_scratch);
-                } else {
-                    for (uint32_t lb = 0; lb < nl; lb += kBlock)
-                        join_staged_pass<W, T>(a, m, g, supp, fold_g, tbits, nl, lb, st, cur0, cur1, scan_scratch);
+            if (overflow) {
+                // Queue the pair for the redo launch, which stages up to the largest pair; in the redo launch
+                // itself an overflow means the host's bound was wrong.
+                if (threadIdx.x == 0) {
+                    if (redo) {
+                        atomicOr(&a.stats[2], 1ull);
+                    } else {
+                        a.redo[atomicAdd(a.redo_count, 1ull)] = p;
+                    }
                 }
-            }
-            // The counts stay per bucket: a total through one global atomic per workgroup would serialize ~10^5
-            // same-address atomics per launch, so k_reduce_lens sums them afterwards.
-            if (threadIdx.x == 0) {
-                a.olen[m.b0] = cur0;
-                if (pair) a.olen[m.b1] = cur1;
+            } else {
+                // ---- phase B over the pair's staged rows ----
+                // One pass in the common case nl <= kBlock: a loop holding stores and no loads would make the
+                // compiler wait for the prefetch before entering it.
+                if (nl != 0) {
+                    __syncthreads();  // the staged rows and the table are complete
+                    if (nl <= kBlock) {
+                        join_staged_pass<W, T>(a, m, g, supp, fold_g, tbits, nl, 0, st, cur0, cur1, scan_scratch);
+                    } else {
+                        for (uint32_t lb = 0; lb < nl; lb += kBlock)
+                            join_staged_pass<W, T>(a, m, g, supp, fold_g, tbits, nl, lb, st, cur0, cur1,
+                                                   scan_scratch);
+                    }
+                }
+                // The counts stay per bucket: a total through one global atomic per workgroup would serialize
+                // ~10^5 same-address atomics per launch, so k_reduce_lens sums them afterwards.
+                if (threadIdx.x == 0) {
+                    a.olen[m.b0] = cur0;
+                    if (pair) a.olen[m.b1] = cur1;
+                }
             }
             if (!have_next) break;
             // The next pair's table: every lookup above precedes phase B's scan barriers, every insert below
             // follows the next scan's barrier.
-            p += G;
+            t += G;
+            p = pn;
             m = mn;
             cur0 = cur1 = nl = 0;
-            tbits = min(table_bits(m.n0 + m.n1), a.tcap_bits);
+            overflow = false;
+            tbits = min(table_bits(min(m.n0 + m.n1, scap)), a.tcap_bits);
             for (uint32_t i = threadIdx.x; i < (1u << tbits); i += kBlock) table[i] = 0u;
         }
         base = nbase;
@@ -676,6 +698,18 @@ hipError_t launch_bucket_offsets(hipStream_t st, const uint32_t* len, uint32_t n
     return hipGetLastError();
 }
 
+// A fill kernel for the counters: a memset node recorded into a captured graph is not replayed reliably.
+__global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, uint32_t n, unsigned long long v) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3((n + 63) / 64), dim3(64), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce_lens(hipStream_t st, const uint32_t* len, uint32_t num_buckets, unsigned long long* out,
                               unsigned long long* prof) {
     const uint32_t blocks = offsets_tiles(num_buckets);
@@ -685,8 +719,8 @@ hipError_t launch_reduce_lens(hipStream_t st, const uint32_t* len, uint32_t num_
 }
 
 template <int W, typename T>
-size_t join_lds_bytes(uint32_t lcap, uint32_t tcap_bits) {
-    return (size_t)lcap * (2 * W * sizeof(uint64_t) + sizeof(T) + sizeof(uint32_t) + 1) + ((size_t)4 << tcap_bits);
+size_t join_lds_bytes(uint32_t scap, uint32_t tcap_bits) {
+    return (size_t)scap * (2 * W * sizeof(uint64_t) + sizeof(T) + sizeof(uint32_t) + 1) + ((size_t)4 << tcap_bits);
 }
 
 // A persistent grid: as many workgroups as stay resident at once (at most groups_per_cu per CU when that is
@@ -694,25 +728,27 @@ size_t join_lds_bytes(uint32_t lcap, uint32_t tcap_bits) {
 template <int W, typename T>
 hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t groups_per_cu) {
     if (a.num_cosets == 0) return hipSuccess;
-    const size_t lds = join_lds_bytes<W, T>(a.lcap, a.tcap_bits);
+    const size_t lds = join_lds_bytes<W, T>(a.scap, a.tcap_bits);
     static int cus = 0;
-    static int resident[17] = {};  // by lcap / 64 (lcap <= 1024, a multiple of 64)
+    static int resident[2][17] = {};  // by launch kind and scap / 64 (scap <= 1024, a multiple of 64)
     if (cus == 0) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
     }
-    int& per = resident[std::min<uint32_t>(a.lcap >> 6, 16u)];
-    if (per == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rotation_join<W, T>, kBlock, lds) !=
-                         hipSuccess ||
-                     per <= 0))
+    const auto kernel = a.redo_mode ? k_rotation_join<W, T, true> : k_rotation_join<W, T, false>;
+    int& per = resident[a.redo_mode ? 1 : 0][std::min<uint32_t>(a.scap >> 6, 16u)];
+    if (per == 0 &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, lds) != hipSuccess || per <= 0))
         per = 1;
     const int per_cu = groups_per_cu ? std::min<int>(per, (int)std::min<uint32_t>(groups_per_cu, 1024u)) : per;
-    const uint32_t grid = groups_per_cu == kJoinOnePairPerGroup
+    // the redo launch does not know its pair count: a persistent grid that reads it on the device
+    const uint32_t grid = a.redo_mode ? (uint32_t)(cus * per_cu)
+                          : groups_per_cu == kJoinOnePairPerGroup
                               ? a.num_cosets
                               : std::min<uint32_t>(a.num_cosets, (uint32_t)(cus * per_cu));
-    hipLaunchKernelGGL((k_rotation_join<W, T>), dim3(grid), dim3(kBlock), lds, st, a);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), lds, st, a);
     return hipGetLastError();
 }
 

@@ -355,6 +355,19 @@ class GpuPauliSum:
         _native.check(_native.lib().pp_apply_rotation_to(self._h, dst._h, ctypes.byref(ch), 1 if adjoint else 0,
                                                          ctypes.byref(pol)), self.ctx.handle)
 
+    def autotune(self, circuit, policy=None) -> "_native.Tuning":
+        """Time the device path's candidate sizes on this sum stepped through ``circuit``'s channels (each out of place
+        from it), keep the fastest in the context and return it; this sum is left as it was."""
+        chans = np.ascontiguousarray(circuit.channels() if isinstance(circuit, Circuit) else circuit)
+        arr = (_native.Channel * max(1, len(chans)))()
+        for i, row in enumerate(chans):
+            arr[i] = self._channel(row)
+        pol = truncation.lower(policy)
+        out = _native.Tuning()
+        _native.check(_native.lib().pp_autotune(self._h, arr, len(chans), ctypes.byref(pol), ctypes.byref(out)),
+                      self.ctx.handle)
+        return out
+
     def truncate_topn(self, n: int) -> None:
         _native.check(_native.lib().pp_truncate_topn(self._h, int(n)), self.ctx.handle)
 

@@ -43,6 +43,16 @@ class Options(ctypes.Structure):
     _fields_ = [("target_bucket_len", ctypes.c_uint32), ("min_buckets", ctypes.c_uint32)]
 
 
+class Tuning(ctypes.Structure):
+    """pp_tuning: the device path's sizes (bucket_target, stage_rows, groups_per_cu)."""
+
+    _fields_ = [("bucket_target", ctypes.c_uint32), ("stage_rows", ctypes.c_uint32),
+                ("groups_per_cu", ctypes.c_uint32)]
+
+
+ONE_PAIR_PER_GROUP = 0xFFFFFFFF
+
+
 class Channel(ctypes.Structure):
     _fields_ = [("gx", ctypes.c_uint64 * 4), ("gz", ctypes.c_uint64 * 4), ("theta", ctypes.c_double)]
 
@@ -89,6 +99,13 @@ def lib() -> ctypes.CDLL:
     L.pp_apply_rotation_to.argtypes = [vp, vp, ctypes.POINTER(Channel), ctypes.c_int, ctypes.POINTER(Policy)]
     L.pp_truncate_topn.argtypes = [vp, u64]
     L.pp_truncate_approx_topn.argtypes = [vp, u64]
+    L.pp_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.pp_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.pp_autotune.argtypes = [vp, ctypes.POINTER(Channel), u64, ctypes.POINTER(Policy), ctypes.POINTER(Tuning)]
+    L.pp_graph_capture_begin.argtypes = [vp]
+    L.pp_graph_capture_end.argtypes = [vp, pvp]
+    L.pp_graph_launch.argtypes = [vp]
+    L.pp_graph_free.argtypes = [vp]
     L.pp_expectation_product.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
     L.pp_profile_enable.argtypes = [vp, ctypes.c_int]
     L.pp_profile_read.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64),
@@ -97,8 +114,10 @@ def lib() -> ctypes.CDLL:
     L.pp_hash_rows.restype = i32
     for name in ("pp_context_create", "pp_sync", "pp_context_stream", "pp_sum_upload", "pp_sum_clone",
                  "pp_sum_free", "pp_sum_len", "pp_sum_download", "pp_sum_bucket_bits", "pp_propagate",
-                 "pp_apply_rotation", "pp_apply_rotation_to", "pp_truncate_topn", "pp_truncate_approx_topn", "pp_expectation_product", "pp_profile_enable",
-                 "pp_profile_read"):
+                 "pp_apply_rotation", "pp_apply_rotation_to", "pp_truncate_topn", "pp_truncate_approx_topn",
+                 "pp_graph_capture_begin", "pp_graph_capture_end", "pp_graph_launch", "pp_graph_free",
+                 "pp_get_tuning", "pp_set_tuning", "pp_autotune",
+                 "pp_expectation_product", "pp_profile_enable", "pp_profile_read"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -154,3 +173,61 @@ class Context:
         check(lib().pp_profile_read(self.handle, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(rin),
                                     ctypes.byref(rout)), self.handle)
         return n.value, ms.value, rin.value, rout.value
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        check(lib().pp_get_tuning(self.handle, ctypes.byref(t)), self.handle)
+        return t
+
+    def set_tuning(self, bucket_target: int, stage_rows: int = 0, groups_per_cu: int = ONE_PAIR_PER_GROUP) -> None:
+        t = Tuning(bucket_target, stage_rows, groups_per_cu)
+        check(lib().pp_set_tuning(self.handle, ctypes.byref(t)), self.handle)
+
+    def capture(self) -> "GraphCapture":
+        """Capture the layers queued inside the ``with`` block into a HIP graph (see pp_graph_capture_begin)."""
+        return GraphCapture(self)
+
+
+class Graph:
+    """A captured layer sequence (pp_graph): ``launch()`` replays it on its context's stream."""
+
+    def __init__(self, ctx: "Context", handle):
+        self.ctx = ctx
+        self._h = handle
+
+    def launch(self) -> None:
+        check(lib().pp_graph_launch(self._h), self.ctx.handle)
+
+    def free(self) -> None:
+        if self._h:
+            check(lib().pp_graph_free(self._h), self.ctx.handle)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class GraphCapture:
+    """``with ctx.capture() as cap: ...`` then ``cap.graph``; an exception inside the block discards the capture."""
+
+    def __init__(self, ctx: "Context"):
+        self.ctx = ctx
+        self.graph: Graph | None = None
+
+    def __enter__(self) -> "GraphCapture":
+        check(lib().pp_graph_capture_begin(self.ctx.handle), self.ctx.handle)
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = ctypes.c_void_p()
+        status = lib().pp_graph_capture_end(self.ctx.handle, ctypes.byref(h))
+        if exc_type is not None:
+            if status == PP_OK and h.value:
+                lib().pp_graph_free(h)
+            return False
+        check(status, self.ctx.handle)
+        self.graph = Graph(self.ctx, h.value)
+        return False

@@ -248,6 +248,35 @@ def test_approx_topn_inside_propagation(dtype):
     assert_same(o, p)
 
 
+@pytest.mark.parametrize("stage_rows", [0, 64])
+@pytest.mark.parametrize("paulis", ["Z", "ZX"])
+def test_every_row_anticommuting(paulis, stage_rows):
+    """Every term anticommutes with the generator, so every bucket pair stages all of its rows; with 64 staging rows
+    that is more than the first join launch holds per workgroup, which leaves every pair to the redo launch."""
+    announce(f"all rows anticommuting {paulis} stage_rows={stage_rows}")
+    from paulistrings_amd import _native
+
+    ctx = _native.Context.get(0)
+    saved = ctx.get_tuning()
+    ctx.set_tuning(saved.bucket_target, stage_rows, saved.groups_per_cu)
+    try:
+        _every_row_anticommuting(paulis)
+    finally:
+        ctx.set_tuning(saved.bucket_target, saved.stage_rows, saved.groups_per_cu)
+
+
+def _every_row_anticommuting(paulis):
+    rng = np.random.default_rng(23)
+    nq, m = 40, 1 << 17
+    x, z, c = random_sum(rng, m, nq, 0.3, F64)
+    x[:, 0] |= np.uint64(1)  # X or Y on qubit 0: anticommutes with Z0 and with Z0 X1 when qubit 1 commutes
+    z[:, 0] &= ~np.uint64(2)
+    x[:, 0] &= ~np.uint64(2)
+    circ = Circuit(nq).pauli_rotation(paulis, [0, 1][: len(paulis)], 0.45)
+    o, p = run_both(nq, x, z, c, circ, "forward", eps=1e-9)
+    assert_same(o, p)
+
+
 def test_edge_cases():
     announce("edge cases")
     # empty sum

@@ -163,7 +163,7 @@ pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms,
 /* The device path's sizes, per context.  No counterpart in the reference, which fixes its device sizes
  * (records_per_block = 4096, target_bucket_len = 1024: engine/gpu/layer/options.rs:11, pauli_sum/storage.rs:54);
  * results do not depend on them.  Defaults come from PP_DEVICE_BUCKET_LEN, PP_JOIN_STAGE_ROWS and
- * PP_JOIN_GROUPS_PER_CU when set. */
+ * PP_JOIN_GROUPS_PER_CU, PP_JOIN_BLOCK when set. */
 typedef struct {
     uint32_t bucket_target; /* rows per device bucket, 16..512 (default 256) */
     uint32_t stage_rows;    /* anticommuting rows a join workgroup stages in its first launch, 64..1024 (a pair
@@ -171,10 +171,12 @@ typedef struct {
                                0: the largest pair, no second launch (default) */
     uint32_t groups_per_cu; /* the join's grid: 0xFFFFFFFF one workgroup per pair (default); 0 a persistent
                                grid of as many as fit; 1..32 persistent with at most that many per CU */
+    uint32_t join_block;    /* threads per join workgroup, the rows of one pass: 256 (default) or 512 */
 } pp_tuning;
 pp_status pp_get_tuning(pp_context* ctx, pp_tuning* out);
 pp_status pp_set_tuning(pp_context* ctx, const pp_tuning* tuning);
-/* Time candidate tunings (bucket_target 128/192/256/384 x stage_rows auto/128/256 x one-per-pair/persistent) on
+/* Time candidate tunings (bucket_target 128/192/256/384 x stage_rows auto/128/256 x one-per-pair/persistent x
+ * join_block 256/512) on
  * `sample`, each channel applied out of place from it as pp_apply_rotation_to does, keep the fastest in the
  * context and return it (out may be NULL).  The sample is left as it was. */
 pp_status pp_autotune(pp_sum* sample, const pp_channel* channels, uint64_t n_channels, const pp_policy* policy,

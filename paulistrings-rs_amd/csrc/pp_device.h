@@ -240,10 +240,10 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 
 // Ordered compaction of NF one-bit flags across the workgroup: ballot + popcount within a wave64, the wave
 // totals through LDS.  `off[i]` is the thread's rank among set flags i, `tot[i]` the block's count.  The wave
-// totals travel packed, two 16-bit fields per word (a block count is at most kBlock), so the cross-wave sums
-// cost one add per word rather than per flag.  `scratch` holds at least NF * kWaves words.  Two barriers;
-// every thread of the block must call it.
-template <int NF>
+// totals travel packed, two 16-bit fields per word (a block count is at most 64 * WAVES), so the cross-wave sums
+// cost one add per word rather than per flag.  `scratch` holds at least NF * WAVES words.  Two barriers;
+// every thread of the block (WAVES wave64s) must call it.
+template <int NF, int WAVES = kWaves>
 __device__ __forceinline__ void block_flag_scan(const bool (&f)[NF], uint32_t (&off)[NF], uint32_t (&tot)[NF],
                                                 uint32_t* scratch) {
     constexpr int NW = (NF + 1) / 2;
@@ -261,7 +261,7 @@ __device__ __forceinline__ void block_flag_scan(const bool (&f)[NF], uint32_t (&
     }
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < NW; ++k) scratch[k * kWaves + wave] = packed[k];
+        for (int k = 0; k < NW; ++k) scratch[k * WAVES + wave] = packed[k];
     }
     __syncthreads();
     uint32_t before[NW], all[NW];
@@ -270,8 +270,8 @@ __device__ __forceinline__ void block_flag_scan(const bool (&f)[NF], uint32_t (&
         before[k] = 0;
         all[k] = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            const uint32_t c = scratch[k * kWaves + w];
+        for (int w = 0; w < WAVES; ++w) {
+            const uint32_t c = scratch[k * WAVES + w];
             if ((uint32_t)w < wave) before[k] += c;
             all[k] += c;
         }

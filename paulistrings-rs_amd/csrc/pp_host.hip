@@ -53,6 +53,12 @@ uint32_t default_stage_rows() {
     return (uint32_t)(t >= 64 && t <= 1024 ? t : 0);
 }
 
+// Threads per join workgroup (PP_JOIN_BLOCK, 256 or 512; default 256): the rows of one pass over a pair.
+uint32_t default_join_block() {
+    const char* e = std::getenv("PP_JOIN_BLOCK");
+    return e && std::strtol(e, nullptr, 10) == 512 ? 512u : 256u;
+}
+
 // Rows per device bucket (PP_DEVICE_BUCKET_LEN, 16..512; default 256).
 uint32_t default_bucket_target() {
     const char* e = std::getenv("PP_DEVICE_BUCKET_LEN");
@@ -62,7 +68,8 @@ uint32_t default_bucket_target() {
 
 bool valid_tuning(const pp_tuning& t) {
     return t.bucket_target >= 16 && t.bucket_target <= 512 && (t.stage_rows == 0 || (t.stage_rows >= 64 && t.stage_rows <= 1024)) &&
-           (t.groups_per_cu <= 32 || t.groups_per_cu == kJoinOnePairPerGroup);
+           (t.groups_per_cu <= 32 || t.groups_per_cu == kJoinOnePairPerGroup) &&
+           (t.join_block == 256 || t.join_block == 512);
 }
 
 // The staging rows of a join whose largest pair is lcap rows: the tuning's (or lcap), a multiple of 64, at most lcap.
@@ -635,12 +642,12 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         a.stats = dst->d_stats;
         a.redo = dst->d_redo;
         a.redo_count = dst->d_stats + 4;
-        hipError_t le = launch_rotation_join<W, T>(ctx->stream, a, ctx->tuning.groups_per_cu);
+        hipError_t le = launch_rotation_join<W, T>(ctx->stream, a, ctx->tuning.groups_per_cu, ctx->tuning.join_block);
         if (le != hipSuccess || scap >= lcap) return le;  // a first launch staging lcap never queues a pair
         a.redo_mode = 1;
         a.scap = lcap;
         a.tcap_bits = table_bits_for(lcap);
-        return launch_rotation_join<W, T>(ctx->stream, a, 0);
+        return launch_rotation_join<W, T>(ctx->stream, a, 0, ctx->tuning.join_block);
     });
     PP_HIP(ctx, e);
     if (timed) {
@@ -734,6 +741,7 @@ pp_status pp_context_create(int device, pp_context** out) {
     ctx->tuning.bucket_target = default_bucket_target();
     ctx->tuning.stage_rows = default_stage_rows();
     ctx->tuning.groups_per_cu = default_groups_per_cu();
+    ctx->tuning.join_block = default_join_block();
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return PP_ERR_DRIVER;
@@ -1337,20 +1345,19 @@ pp_status pp_autotune(pp_sum* sample, const pp_channel* channels, uint64_t n_cha
         if (a) pp_sum_free(a);
         return rs;
     };
-    const uint32_t targets[] = {128, 192, 256, 384};
-    const uint32_t stages[] = {0, 128, 256};
-    const uint32_t grids[] = {kJoinOnePairPerGroup, 0};
-    for (uint32_t bt : targets)
-        for (uint32_t sr : stages)
-            for (uint32_t gp : grids) {
-                pp_tuning t{bt, sr, gp};
-                double ms = 0;
-                if ((st = trial(t, &ms)) != PP_OK) break;
-                if (best_ms < 0 || ms < best_ms) {
-                    best_ms = ms;
-                    best = t;
-                }
-            }
+    std::vector<pp_tuning> candidates;
+    for (uint32_t bt : {128u, 192u, 256u, 384u})
+        for (uint32_t sr : {0u, 128u, 256u})
+            for (uint32_t gp : {kJoinOnePairPerGroup, 0u})
+                for (uint32_t jb : {256u, 512u}) candidates.push_back(pp_tuning{bt, sr, gp, jb});
+    for (const pp_tuning& t : candidates) {
+        double ms = 0;
+        if ((st = trial(t, &ms)) != PP_OK) break;
+        if (best_ms < 0 || ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ctx->profiling = profiling;

@@ -146,8 +146,9 @@ size_t join_lds_bytes(uint32_t scap, uint32_t tcap_bits);
 // groups_per_cu caps the resident workgroups per CU of the persistent grid (0: as many as fit;
 // kJoinOnePairPerGroup: a workgroup per pair, no persistence).
 constexpr uint32_t kJoinOnePairPerGroup = 0xFFFFFFFFu;
+// block: 256 or 512 threads per workgroup (a chunk of that many rows per pass).
 template <int W, typename T>
-hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t groups_per_cu);
+hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t groups_per_cu, uint32_t block);
 template <int W, typename T>
 hipError_t launch_rescale(hipStream_t st, const RescaleArgs<W, T>& a, uint32_t num_buckets);
 template <int W, typename T>

@@ -256,7 +256,7 @@ __device__ __forceinline__ uint32_t table_bits(uint32_t n) {
     return t;
 }
 
-// Phase B over staged rows lb .. lb + kBlock - 1 of a pair (see k_rotation_join).
+// Phase B over staged rows lb .. lb + BLOCK - 1 of a pair (see k_rotation_join).
 template <int W, typename T>
 struct Staged {
     uint64_t* x;  // [W][stride]
@@ -268,7 +268,7 @@ struct Staged {
     uint32_t stride;  // staged rows held (the word planes' stride)
 };
 
-template <int W, typename T>
+template <int W, typename T, int BLOCK>
 __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const PairMeta& m, const Key<W>& g,
                                                  uint32_t supp, uint32_t fold_g, uint32_t tbits, uint32_t nl,
                                                  uint32_t lb, const Staged<W, T>& st, uint32_t& cur0,
@@ -331,7 +331,7 @@ __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const 
     const uint32_t jn = pair ? (j ^ 1u) : j;
     bool f[4] = {keep_own && j == 0, keep_own && j == 1, keep_new && jn == 0, keep_new && jn == 1};
     uint32_t off[4], tot[4];
-    block_flag_scan<4>(f, off, tot, scan_scratch);
+    block_flag_scan<4, BLOCK / 64>(f, off, tot, scan_scratch);
     // one store sequence per output kind, for either bucket
     if (keep_own) {
         const uint64_t pos = j == 0 ? m.o0 + cur0 + off[0] : m.o1 + cur1 + off[1];
@@ -349,10 +349,10 @@ __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const 
 
 // REDO: the launch over the queued pairs (a compile-time switch, so that the first launch's pair index needs no
 // load: a lookup there, merged with the prefetch's path, would make the compiler wait for the prefetch).
-template <int W, typename T, bool REDO>
-__global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a) {
+template <int W, typename T, bool REDO, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a) {
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ uint32_t scan_scratch[4 * kWaves];
+    __shared__ uint32_t scan_scratch[4 * (BLOCK / 64)];
     const uint32_t scap = a.scap;  // staged rows this launch holds
     uint64_t* const Lx = reinterpret_cast<uint64_t*>(lds);   // [W][scap]
     uint64_t* const Lz = Lx + (size_t)W * scap;              // [W][scap]
@@ -391,14 +391,14 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
     uint32_t base = 0, cur0 = 0, cur1 = 0, nl = 0;
     bool overflow = false;  // more anticommuting rows than scap: the pair is left to the redo launch
     uint32_t tbits = min(table_bits(min(m.n0 + m.n1, scap)), a.tcap_bits);
-    for (uint32_t i = threadIdx.x; i < (1u << tbits); i += kBlock) table[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < (1u << tbits); i += BLOCK) table[i] = 0u;
     for (;;) {
         const uint32_t n = m.n0 + m.n1;
         // ---- the next chunk: this pair's, or the first of the next pair ----
-        const bool last_chunk = base + kBlock >= n;
+        const bool last_chunk = base + BLOCK >= n;
         bool have_next = true;
         PairMeta mn = m;
-        uint32_t nbase = base + kBlock;
+        uint32_t nbase = base + BLOCK;
         uint32_t pn = p;
         if (last_chunk) {
             if (t + G <= last) {
@@ -425,7 +425,7 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
             }
             bool f[3] = {keep && j == 0, keep && j == 1, anti};
             uint32_t off[3], tot[3];
-            block_flag_scan<3>(f, off, tot, scan_scratch);
+            block_flag_scan<3, BLOCK / 64>(f, off, tot, scan_scratch);
             overflow = nl + tot[2] > scap;
             if (keep) {  // one store sequence for either bucket (a redo rewrites the same positions)
                 const uint64_t pos = j == 0 ? m.o0 + cur0 + off[0] : m.o1 + cur1 + off[1];
@@ -468,15 +468,15 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
                 }
             } else {
                 // ---- phase B over the pair's staged rows ----
-                // One pass in the common case nl <= kBlock: a loop holding stores and no loads would make the
+                // One pass in the common case nl <= BLOCK: a loop holding stores and no loads would make the
                 // compiler wait for the prefetch before entering it.
                 if (nl != 0) {
                     __syncthreads();  // the staged rows and the table are complete
-                    if (nl <= kBlock) {
-                        join_staged_pass<W, T>(a, m, g, supp, fold_g, tbits, nl, 0, st, cur0, cur1, scan_scratch);
+                    if (nl <= BLOCK) {
+                        join_staged_pass<W, T, BLOCK>(a, m, g, supp, fold_g, tbits, nl, 0, st, cur0, cur1, scan_scratch);
                     } else {
-                        for (uint32_t lb = 0; lb < nl; lb += kBlock)
-                            join_staged_pass<W, T>(a, m, g, supp, fold_g, tbits, nl, lb, st, cur0, cur1,
+                        for (uint32_t lb = 0; lb < nl; lb += BLOCK)
+                            join_staged_pass<W, T, BLOCK>(a, m, g, supp, fold_g, tbits, nl, lb, st, cur0, cur1,
                                                    scan_scratch);
                     }
                 }
@@ -496,7 +496,7 @@ __global__ void __launch_bounds__(kBlock) k_rotation_join(const JoinArgs<W, T> a
             cur0 = cur1 = nl = 0;
             overflow = false;
             tbits = min(table_bits(min(m.n0 + m.n1, scap)), a.tcap_bits);
-            for (uint32_t i = threadIdx.x; i < (1u << tbits); i += kBlock) table[i] = 0u;
+            for (uint32_t i = threadIdx.x; i < (1u << tbits); i += BLOCK) table[i] = 0u;
         }
         base = nbase;
         cb = nb;
@@ -726,21 +726,23 @@ size_t join_lds_bytes(uint32_t scap, uint32_t tcap_bits) {
 // A persistent grid: as many workgroups as stay resident at once (at most groups_per_cu per CU when that is
 // nonzero), never more than there are pairs; kJoinOnePairPerGroup launches one workgroup per pair instead.
 template <int W, typename T>
-hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t groups_per_cu) {
+hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t groups_per_cu, uint32_t block) {
     if (a.num_cosets == 0) return hipSuccess;
     const size_t lds = join_lds_bytes<W, T>(a.scap, a.tcap_bits);
     static int cus = 0;
-    static int resident[2][17] = {};  // by launch kind and scap / 64 (scap <= 1024, a multiple of 64)
+    static int resident[2][2][17] = {};  // by block, launch kind and scap / 64 (scap <= 1024, a multiple of 64)
     if (cus == 0) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
     }
-    const auto kernel = a.redo_mode ? k_rotation_join<W, T, true> : k_rotation_join<W, T, false>;
-    int& per = resident[a.redo_mode ? 1 : 0][std::min<uint32_t>(a.scap >> 6, 16u)];
+    const bool wide = block == 512;
+    const auto kernel = a.redo_mode ? (wide ? k_rotation_join<W, T, true, 512> : k_rotation_join<W, T, true, 256>)
+                                    : (wide ? k_rotation_join<W, T, false, 512> : k_rotation_join<W, T, false, 256>);
+    int& per = resident[wide ? 1 : 0][a.redo_mode ? 1 : 0][std::min<uint32_t>(a.scap >> 6, 16u)];
     if (per == 0 &&
-        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, lds) != hipSuccess || per <= 0))
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, lds) != hipSuccess || per <= 0))
         per = 1;
     const int per_cu = groups_per_cu ? std::min<int>(per, (int)std::min<uint32_t>(groups_per_cu, 1024u)) : per;
     // the redo launch does not know its pair count: a persistent grid that reads it on the device
@@ -748,7 +750,7 @@ hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_
                           : groups_per_cu == kJoinOnePairPerGroup
                               ? a.num_cosets
                               : std::min<uint32_t>(a.num_cosets, (uint32_t)(cus * per_cu));
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), lds, st, a);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, st, a);
     return hipGetLastError();
 }
 
@@ -788,7 +790,7 @@ hipError_t launch_export(hipStream_t st, const ExportArgs<W, T>& a, uint32_t num
 
 #define PP_INSTANTIATE(W, T)                                                                           \
     template size_t join_lds_bytes<W, T>(uint32_t, uint32_t);                                          \
-    template hipError_t launch_rotation_join<W, T>(hipStream_t, const JoinArgs<W, T>&, uint32_t);       \
+    template hipError_t launch_rotation_join<W, T>(hipStream_t, const JoinArgs<W, T>&, uint32_t, uint32_t); \
     template hipError_t launch_rescale<W, T>(hipStream_t, const RescaleArgs<W, T>&, uint32_t);          \
     template hipError_t launch_refine<W, T>(hipStream_t, const RefineArgs<W, T>&);                      \
     template hipError_t launch_coarsen<W, T>(hipStream_t, const CoarsenArgs<W, T>&);                    \

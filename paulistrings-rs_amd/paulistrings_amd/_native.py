@@ -44,10 +44,10 @@ class Options(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    """pp_tuning: the device path's sizes (bucket_target, stage_rows, groups_per_cu)."""
+    """pp_tuning: the device path's sizes (bucket_target, stage_rows, groups_per_cu, join_block)."""
 
     _fields_ = [("bucket_target", ctypes.c_uint32), ("stage_rows", ctypes.c_uint32),
-                ("groups_per_cu", ctypes.c_uint32)]
+                ("groups_per_cu", ctypes.c_uint32), ("join_block", ctypes.c_uint32)]
 
 
 ONE_PAIR_PER_GROUP = 0xFFFFFFFF
@@ -179,8 +179,9 @@ class Context:
         check(lib().pp_get_tuning(self.handle, ctypes.byref(t)), self.handle)
         return t
 
-    def set_tuning(self, bucket_target: int, stage_rows: int = 0, groups_per_cu: int = ONE_PAIR_PER_GROUP) -> None:
-        t = Tuning(bucket_target, stage_rows, groups_per_cu)
+    def set_tuning(self, bucket_target: int, stage_rows: int = 0, groups_per_cu: int = ONE_PAIR_PER_GROUP,
+                   join_block: int = 256) -> None:
+        t = Tuning(bucket_target, stage_rows, groups_per_cu, join_block)
         check(lib().pp_set_tuning(self.handle, ctypes.byref(t)), self.handle)
 
     def capture(self) -> "GraphCapture":

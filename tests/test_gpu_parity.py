@@ -258,11 +258,11 @@ def test_every_row_anticommuting(paulis, stage_rows):
 
     ctx = _native.Context.get(0)
     saved = ctx.get_tuning()
-    ctx.set_tuning(saved.bucket_target, stage_rows, saved.groups_per_cu)
+    ctx.set_tuning(saved.bucket_target, stage_rows, saved.groups_per_cu, saved.join_block)
     try:
         _every_row_anticommuting(paulis)
     finally:
-        ctx.set_tuning(saved.bucket_target, saved.stage_rows, saved.groups_per_cu)
+        ctx.set_tuning(saved.bucket_target, saved.stage_rows, saved.groups_per_cu, saved.join_block)
 
 
 def _every_row_anticommuting(paulis):

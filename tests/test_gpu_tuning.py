@@ -16,10 +16,12 @@ from paulistrings_amd import Circuit, PauliSum, _native, truncation
 pytestmark = pytest.mark.gpu
 
 TUNINGS = [
-    (256, 0, _native.ONE_PAIR_PER_GROUP),
-    (64, 64, _native.ONE_PAIR_PER_GROUP),  # small staging: many pairs take the redo launch
-    (384, 128, 0),
-    (128, 256, 3),
+    (256, 0, _native.ONE_PAIR_PER_GROUP, 256),
+    (64, 64, _native.ONE_PAIR_PER_GROUP, 256),  # small staging: many pairs take the redo launch
+    (384, 128, 0, 512),
+    (128, 256, 3, 256),
+    (256, 0, _native.ONE_PAIR_PER_GROUP, 512),
+    (512, 64, 2, 512),
 ]
 
 
@@ -71,7 +73,7 @@ def test_tunings_give_the_oracle_terms():
             assert np.array_equal(h.x_array(), ox) and np.array_equal(h.z_array(), oz), t
             assert np.array_equal(h.coefficients_array(), oc), t
     finally:
-        ctx.set_tuning(saved.bucket_target, saved.stage_rows, saved.groups_per_cu)
+        ctx.set_tuning(saved.bucket_target, saved.stage_rows, saved.groups_per_cu, saved.join_block)
 
 
 def test_autotune_picks_a_candidate_and_keeps_the_sample():
@@ -87,7 +89,7 @@ def test_autotune_picks_a_candidate_and_keeps_the_sample():
     try:
         t = g.autotune(Circuit(nq).rzz(0.9, 0, 1).rx(0.4, 77), truncation.coeff(1e-6))
         assert t.bucket_target in (128, 192, 256, 384) and t.stage_rows in (0, 128, 256)
-        assert t.groups_per_cu in (0, _native.ONE_PAIR_PER_GROUP)
+        assert t.groups_per_cu in (0, _native.ONE_PAIR_PER_GROUP) and t.join_block in (256, 512)
         got = ctx.get_tuning()
         assert (got.bucket_target, got.stage_rows, got.groups_per_cu) == (t.bucket_target, t.stage_rows,
                                                                           t.groups_per_cu)
@@ -95,5 +97,5 @@ def test_autotune_picks_a_candidate_and_keeps_the_sample():
         assert np.array_equal(after.x_array(), before.x_array())
         assert np.array_equal(after.coefficients_array(), before.coefficients_array())
     finally:
-        ctx.set_tuning(saved.bucket_target, saved.stage_rows, saved.groups_per_cu)
+        ctx.set_tuning(saved.bucket_target, saved.stage_rows, saved.groups_per_cu, saved.join_block)
         g.free()

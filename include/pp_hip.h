@@ -158,6 +158,20 @@ pp_status pp_profile_enable(pp_context* ctx, int enable);
 pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms, uint64_t* rows_in,
                           uint64_t* rows_out);
 
+/* ---- batches ------------------------------------------------------------------------------------ */
+
+/* Many independent propagations stepped together (no counterpart in the reference: each of its instances is one
+ * propagate call, engine/mod.rs:121-132).  A batch holds sums of one context, one width and one coefficient type;
+ * freeing it leaves the sums.  pp_batch_propagate steps every sum i through its own circuit
+ * channels[offsets[i] .. offsets[i + 1]) in directions[i] under policies[i] (NULL: keep every nonzero term),
+ * each with exactly pp_propagate's result; per step, one launch of each batched kernel serves every sum that
+ * takes a rotation join (a key-preserving layer, a refinement or a layer pass runs on its sum alone). */
+typedef struct pp_batch pp_batch;
+pp_status pp_batch_create(pp_context* ctx, pp_sum* const* sums, uint32_t count, pp_batch** out);
+pp_status pp_batch_free(pp_batch* batch);
+pp_status pp_batch_propagate(pp_batch* batch, const pp_channel* channels, const uint64_t* offsets,
+                             const pp_direction* directions, const pp_policy* policies, const pp_options* options);
+
 /* ---- tuning ------------------------------------------------------------------------------------- */
 
 /* The device path's sizes, per context.  No counterpart in the reference, which fixes its device sizes

@@ -717,6 +717,76 @@ pp_status apply_one(pp_sum* s, const pp_channel& ch, const Prepared& p, const Ke
 pp_status topn_device(pp_sum* s, uint64_t n);         // pp_topn_impl.inc
 pp_status approx_topn_device(pp_sum* s, uint64_t n);  // pp_topn_impl.inc
 
+pp_status check_channels(pp_sum* s, const pp_channel* channels, uint64_t n_channels) {
+    for (uint64_t i = 0; i < n_channels; ++i)
+        for (uint32_t w = 0; w < 4; ++w) {
+            const uint64_t live = w < s->W ? live_mask(s->num_qubits, w) : 0;
+            if ((channels[i].gx[w] | channels[i].gz[w]) & ~live)
+                return fail(s->ctx, PP_ERR_INVALID_ARGUMENT,
+                            "channel " + std::to_string(i) + " acts outside the " + std::to_string(s->num_qubits) +
+                                "-qubit register");
+        }
+    return PP_OK;
+}
+
+// run_layers' schedule (engine/driver.rs:319-592) for one sum: the reference's bucket count moves at the start of
+// every run (refine_unprepared, engine/backend.rs:155-165), Runs::take (engine/driver.rs:202-258) joins
+// permutation steps or rotation tables into runs that share that count, and the layer pass runs once after each.
+struct RunPlanner {
+    const pp_channel* channels;
+    uint64_t n_channels;
+    bool adjoint;
+    const pp_policy* policy;
+    size_t target, min_buckets;
+    bool finalizes = false;
+    uint64_t limit = UINT64_MAX;  // run_length_limit of TopN and ApproxTopN (builtin.rs:147-149, :462-464)
+
+    RunPlanner(const pp_channel* c, uint64_t n, bool adj, const pp_policy* pol, size_t t, size_t mb)
+        : channels(c), n_channels(n), adjoint(adj), policy(pol), target(t), min_buckets(mb) {
+        finalizes = pol && pol->has_topn;
+        limit = finalizes ? pol->topn : UINT64_MAX;
+    }
+
+    const pp_channel& channel_at(uint64_t i) const { return channels[adjoint ? n_channels - 1 - i : i]; }
+
+    // The reference's count for the run starting at layer i, and the run's end.
+    uint64_t start_run(pp_sum* s, uint64_t i) const {
+        const int want = rebucketed_bits(s->len, s->reference_bits, target, min_buckets);
+        s->reference_bits = std::min(want, s->max_bits);
+        const Prepared p = prepare(s, channel_at(i), adjoint);
+        uint64_t end = i + 1;
+        if (limit == 0) return end;
+        if (s->len <= limit && is_permutation_step(p)) {
+            uint64_t j = i + 1;
+            while (j < n_channels && is_permutation_step(prepare(s, channel_at(j), adjoint))) ++j;
+            if (j > i + 1) return j;
+        }
+        auto mul = [](uint64_t a, uint64_t b) { return (a != 0 && b > UINT64_MAX / a) ? UINT64_MAX : a * b; };
+        uint64_t worst = mul(s->len, fanout(p));
+        if (worst <= limit && is_rotation_table(p)) {
+            std::vector<uint32_t> deltas{0, bucket_of_host(s, channel_at(i).gx, channel_at(i).gz, s->reference_bits)};
+            uint64_t j = i + 1;
+            while (j < n_channels && j < i + kRunLayers) {
+                const Prepared q = prepare(s, channel_at(j), adjoint);
+                const uint64_t grown = mul(worst, fanout(q));
+                if (grown > limit || !is_rotation_table(q)) break;
+                auto next = deltas;
+                next.push_back(0);
+                next.push_back(bucket_of_host(s, channel_at(j).gx, channel_at(j).gz, s->reference_bits));
+                if (span_dimension(next) > kRunDimension) break;
+                deltas = next;
+                worst = grown;
+                ++j;
+            }
+            if (j > i + 1) end = j;
+        }
+        return end;
+    }
+
+    // The layer pass after a run.
+    pp_status finish_run(pp_sum* s) const;
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1107,76 +1177,20 @@ pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channel
     const size_t target = options ? options->target_bucket_len : 1024;
     const size_t min_buckets = options ? options->min_buckets : 128;
     if (target < 1 || min_buckets < 1) return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "bucket options");
-    for (uint64_t i = 0; i < n_channels; ++i)
-        for (uint32_t w = 0; w < 4; ++w) {
-            const uint64_t live = w < s->W ? live_mask(s->num_qubits, w) : 0;
-            if ((channels[i].gx[w] | channels[i].gz[w]) & ~live)
-                return fail(s->ctx, PP_ERR_INVALID_ARGUMENT,
-                            "channel " + std::to_string(i) + " acts outside the " + std::to_string(s->num_qubits) +
-                                "-qubit register");
-        }
-    const bool adjoint = direction == PP_HEISENBERG;
-    const KeepProgram keep = lower_policy(policy);
-    const bool finalizes = policy && policy->has_topn;
-    // run_length_limit of TopN and ApproxTopN (builtin.rs:147-149, :462-464)
-    const uint64_t limit = finalizes ? policy->topn : UINT64_MAX;
-    auto channel_at = [&](uint64_t i) -> const pp_channel& {
-        return channels[adjoint ? n_channels - 1 - i : i];
-    };
     pp_status st;
+    if ((st = check_channels(s, channels, n_channels)) != PP_OK) return st;
+    const RunPlanner plan{channels, n_channels, direction == PP_HEISENBERG, policy, target, min_buckets};
+    const KeepProgram keep = lower_policy(policy);
     uint64_t i = 0;
     while (i < n_channels) {
-        // refine_unprepared (engine/backend.rs:155-165): the reference's count moves before the layer
-        int want = rebucketed_bits(s->len, s->reference_bits, target, min_buckets);
-        s->reference_bits = std::min(want, s->max_bits);
-        const Prepared p = prepare(s, channel_at(i), adjoint);
-        // Runs::take (engine/driver.rs:202-258): layers joined into a run share this bucket count
-        uint64_t end = i + 1;
-        bool run = false;
-        if (limit > 0) {
-            if (s->len <= limit && is_permutation_step(p)) {
-                uint64_t j = i + 1;
-                while (j < n_channels && is_permutation_step(prepare(s, channel_at(j), adjoint))) ++j;
-                if (j > i + 1) {
-                    end = j;
-                    run = true;
-                }
-            }
-            if (!run) {
-                auto mul = [](uint64_t a, uint64_t b) {
-                    return (a != 0 && b > UINT64_MAX / a) ? UINT64_MAX : a * b;
-                };
-                uint64_t worst = mul(s->len, fanout(p));
-                if (worst <= limit && is_rotation_table(p)) {
-                    std::vector<uint32_t> deltas{0, bucket_of_host(s, channel_at(i).gx, channel_at(i).gz,
-                                                                   s->reference_bits)};
-                    uint64_t j = i + 1;
-                    while (j < n_channels && j < i + kRunLayers) {
-                        const Prepared q = prepare(s, channel_at(j), adjoint);
-                        const uint64_t grown = mul(worst, fanout(q));
-                        if (grown > limit || !is_rotation_table(q)) break;
-                        auto next = deltas;
-                        next.push_back(0);
-                        next.push_back(bucket_of_host(s, channel_at(j).gx, channel_at(j).gz, s->reference_bits));
-                        if (span_dimension(next) > kRunDimension) break;
-                        deltas = next;
-                        worst = grown;
-                        ++j;
-                    }
-                    if (j > i + 1) end = j;
-                }
-            }
-        }
+        const uint64_t end = plan.start_run(s, i);
         for (uint64_t j = i; j < end; ++j) {
-            const Prepared q = j == i ? p : prepare(s, channel_at(j), adjoint);
-            if ((st = apply_one(s, channel_at(j), q, keep)) != PP_OK) return st;
+            const Prepared q = prepare(s, plan.channel_at(j), plan.adjoint);
+            if ((st = apply_one(s, plan.channel_at(j), q, keep)) != PP_OK) return st;
             if (layer_len) layer_len[j] = s->len;
         }
-        if (finalizes) {
-            st = policy->has_topn == 2 ? pp_truncate_approx_topn(s, policy->topn) : pp_truncate_topn(s, policy->topn);
-            if (st != PP_OK) return st;
-            if (layer_len) layer_len[end - 1] = s->len;
-        }
+        if ((st = plan.finish_run(s)) != PP_OK) return st;
+        if (layer_len && plan.finalizes) layer_len[end - 1] = s->len;
         i = end;
     }
     return PP_OK;
@@ -1198,6 +1212,11 @@ pp_status pp_truncate_approx_topn(pp_sum* s, uint64_t n) {
     if (rs != PP_OK) return rs;
     if (s->len <= n) return PP_OK;  // EdgeDecision::KeepAll
     return approx_topn_device(s, n);
+}
+
+pp_status RunPlanner::finish_run(pp_sum* s) const {
+    if (!finalizes) return PP_OK;
+    return policy->has_topn == 2 ? pp_truncate_approx_topn(s, policy->topn) : pp_truncate_topn(s, policy->topn);
 }
 
 pp_status pp_expectation_product(pp_sum* s, const uint64_t* axis_x, const uint64_t* axis_z, const uint64_t* negative,
@@ -1287,6 +1306,223 @@ pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms,
 }
 
 }  // extern "C"
+
+// ---- batches: many sums stepped together, one join launch per layer step -----------------------------------------
+
+struct pp_batch {
+    pp_context* ctx = nullptr;
+    std::vector<pp_sum*> sums;
+    uint32_t W = 1;
+    pp_coeff_kind kind = PP_COEFF_F64;
+    void* d_items = nullptr;  // JoinArgs per item of a step
+    void* h_items = nullptr;  // pinned staging of the same
+    uint32_t* d_pair_start = nullptr;
+    uint32_t* h_pair_start = nullptr;
+    unsigned long long* d_out = nullptr;  // 4 counters per item
+    unsigned long long* h_out = nullptr;
+};
+
+pp_status pp_batch_create(pp_context* ctx, pp_sum* const* sums, uint32_t count, pp_batch** out) {
+    if (!ctx || !out || (count && !sums)) return PP_ERR_INVALID_ARGUMENT;
+    PP_NOT_CAPTURING(ctx, "pp_batch_create");
+    *out = nullptr;
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!sums[i] || sums[i]->ctx != ctx) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "batch: a sum of another context");
+        if (sums[i]->W != sums[0]->W || sums[i]->kind != sums[0]->kind)
+            return fail(ctx, PP_ERR_UNSUPPORTED, "batch: the sums differ in width or coefficient type");
+        for (uint32_t j = 0; j < i; ++j)
+            if (sums[j] == sums[i]) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "batch: a sum appears twice");
+    }
+    auto* b = new (std::nothrow) pp_batch();
+    if (!b) return fail(ctx, PP_ERR_OUT_OF_MEMORY, "host allocation");
+    b->ctx = ctx;
+    b->sums.assign(sums, sums + count);
+    if (count) {
+        b->W = sums[0]->W;
+        b->kind = sums[0]->kind;
+    }
+    const size_t item_bytes = b->W == 1 ? (b->kind == PP_COEFF_F64 ? sizeof(JoinArgs<1, double>) : sizeof(JoinArgs<1, float>))
+                                        : (b->kind == PP_COEFF_F64 ? sizeof(JoinArgs<2, double>) : sizeof(JoinArgs<2, float>));
+    const size_t n = std::max<uint32_t>(count, 1);
+    pp_status st;
+    auto bail = [&](pp_status e) {
+        pp_batch_free(b);
+        return e;
+    };
+    if ((st = dev_alloc(ctx, &b->d_items, n * item_bytes)) != PP_OK ||
+        (st = dev_alloc(ctx, (void**)&b->d_pair_start, (n + 1) * 4)) != PP_OK ||
+        (st = dev_alloc(ctx, (void**)&b->d_out, n * 4 * sizeof(unsigned long long))) != PP_OK)
+        return bail(st);
+    if (hipHostMalloc(&b->h_items, n * item_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_pair_start, (n + 1) * 4, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_out, n * 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned host allocation"));
+    *out = b;
+    return PP_OK;
+}
+
+pp_status pp_batch_free(pp_batch* b) {
+    if (!b) return PP_OK;
+    PP_NOT_CAPTURING(b->ctx, "pp_batch_free");
+    (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_pair_start) (void)hipFree(b->d_pair_start);
+    if (b->d_out) (void)hipFree(b->d_out);
+    if (b->h_items) (void)hipHostFree(b->h_items);
+    if (b->h_pair_start) (void)hipHostFree(b->h_pair_start);
+    if (b->h_out) (void)hipHostFree(b->h_out);
+    delete b;
+    return PP_OK;
+}
+
+namespace {
+
+// One layer step of a batch: item k joins sums[idx[k]] through chans[k] into its spare column set, all in one
+// launch of each batched kernel, then one read-back of every item's counters.
+template <int W, typename T>
+pp_status batch_step(pp_batch* b, const std::vector<uint32_t>& idx, const std::vector<pp_channel>& chans,
+                     const std::vector<Prepared>& preps, const std::vector<KeepProgram>& keeps) {
+    pp_context* ctx = b->ctx;
+    pp_status st;
+    auto* items = (JoinArgs<W, T>*)b->h_items;
+    uint32_t total_pairs = 0, scap_max = 64, tbits_max = 1;
+    for (size_t k = 0; k < idx.size(); ++k) {
+        pp_sum* s = b->sums[idx[k]];
+        const int in = s->cur, out = 1 - s->cur;
+        const uint32_t B = 1u << s->device_bits;
+        if ((st = ensure_columns(s, out, 2 * s->len + (uint64_t)kSegAlign * B)) != PP_OK) return st;
+        if ((st = ensure_buckets(s, out, B)) != PP_OK) return st;
+        const pp_channel& ch = chans[k];
+        const uint32_t delta = bucket_of_host(s, ch.gx, ch.gz, s->device_bits);
+        uint32_t lcap = delta ? 2u * s->max_bucket : s->max_bucket;
+        lcap = std::max<uint32_t>(64, (lcap + 63) & ~63u);
+        if (lcap > kMaxPairRows) lcap = kMaxPairRows;
+        uint32_t tb = 1;
+        while ((1u << tb) < 2u * lcap) ++tb;
+        JoinArgs<W, T> a{};
+        a.x = s->col[in].x;
+        a.z = s->col[in].z;
+        a.c = (const T*)s->col[in].c;
+        a.cap = s->col[in].cap;
+        a.start = s->bk[in].start;
+        a.len = s->bk[in].len;
+        a.ox = s->col[out].x;
+        a.oz = s->col[out].z;
+        a.oc = (T*)s->col[out].c;
+        a.ocap = s->col[out].cap;
+        a.ostart = s->bk[out].start;
+        a.olen = s->bk[out].len;
+        a.delta = delta;
+        a.hb = delta ? 31u - (uint32_t)__builtin_clz(delta) : 0u;
+        a.scap = lcap;
+        a.tcap_bits = tb;
+        a.num_cosets = delta ? B / 2 : B;
+        uint32_t gphase = 0;
+        for (uint32_t w = 0; w < W; ++w) {
+            a.gx[w] = ch.gx[w];
+            a.gz[w] = ch.gz[w];
+            gphase += (uint32_t)__builtin_popcountll(ch.gx[w] & ch.gz[w]);
+        }
+        a.gphase = gphase & 3u;
+        a.cos_t = (T)preps[k].cos;
+        a.sin_t = (T)preps[k].sin;
+        a.clifford = preps[k].clifford ? 1 : 0;
+        a.keep = keeps[k];
+        a.stats = s->d_stats;
+        a.redo = nullptr;  // every item stages its largest pair
+        items[k] = a;
+        b->h_pair_start[k] = total_pairs;
+        total_pairs += a.num_cosets;
+        scap_max = std::max(scap_max, lcap);
+        tbits_max = std::max(tbits_max, tb);
+    }
+    const uint32_t n = (uint32_t)idx.size();
+    b->h_pair_start[n] = total_pairs;
+    PP_HIP(ctx, hipMemcpyAsync(b->d_items, items, n * sizeof(JoinArgs<W, T>), hipMemcpyHostToDevice, ctx->stream));
+    PP_HIP(ctx, hipMemcpyAsync(b->d_pair_start, b->h_pair_start, (n + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    const hipError_t le = launch_join_batch<W, T>(ctx->stream, (const JoinArgs<W, T>*)b->d_items, b->d_pair_start,
+                                                  n, total_pairs, scap_max, tbits_max, b->d_out);
+    PP_HIP(ctx, le);
+    PP_HIP(ctx, hipMemcpyAsync(b->h_out, b->d_out, n * 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t k = 0; k < idx.size(); ++k) {
+        pp_sum* s = b->sums[idx[k]];
+        if (b->h_out[4 * k + 2]) return fail(ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
+        s->len = b->h_out[4 * k];
+        s->max_bucket = (uint32_t)b->h_out[4 * k + 1];
+        s->cur = 1 - s->cur;
+    }
+    return PP_OK;
+}
+
+}  // namespace
+
+pp_status pp_batch_propagate(pp_batch* b, const pp_channel* channels, const uint64_t* offsets,
+                             const pp_direction* directions, const pp_policy* policies, const pp_options* options) {
+    if (!b || !offsets || !directions) return PP_ERR_INVALID_ARGUMENT;
+    pp_context* ctx = b->ctx;
+    PP_NOT_CAPTURING(ctx, "pp_batch_propagate");
+    const uint32_t count = (uint32_t)b->sums.size();
+    const size_t target = options ? options->target_bucket_len : 1024;
+    const size_t min_buckets = options ? options->min_buckets : 128;
+    if (target < 1 || min_buckets < 1) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "bucket options");
+    pp_status st;
+    std::vector<RunPlanner> plans;
+    std::vector<KeepProgram> keeps(count);
+    plans.reserve(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        pp_sum* s = b->sums[i];
+        if (offsets[i + 1] < offsets[i] || (offsets[i + 1] > offsets[i] && !channels))
+            return fail(ctx, PP_ERR_INVALID_ARGUMENT, "batch: channel offsets");
+        const pp_policy* pol = policies ? &policies[i] : nullptr;
+        if (!valid_policy(pol)) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
+        if ((st = refresh(s)) != PP_OK) return st;
+        if ((st = check_channels(s, channels + offsets[i], offsets[i + 1] - offsets[i])) != PP_OK) return st;
+        plans.emplace_back(channels + offsets[i], offsets[i + 1] - offsets[i], directions[i] == PP_HEISENBERG, pol,
+                           target, min_buckets);
+        keeps[i] = lower_policy(pol);
+    }
+    std::vector<uint64_t> next(count, 0), run_end(count, 0);
+    for (;;) {
+        std::vector<uint32_t> idx;
+        std::vector<pp_channel> chans;
+        std::vector<Prepared> preps;
+        std::vector<KeepProgram> ks;
+        for (uint32_t i = 0; i < count; ++i) {
+            const RunPlanner& plan = plans[i];
+            if (next[i] >= plan.n_channels) continue;
+            pp_sum* s = b->sums[i];
+            if (next[i] == run_end[i]) run_end[i] = plan.start_run(s, next[i]);
+            const pp_channel& ch = plan.channel_at(next[i]);
+            if ((st = settle_device_bits(s)) != PP_OK) return st;
+            const Prepared p = prepare(s, ch, plan.adjoint);
+            if (p.key_preserving) {  // in place, on its own
+                if ((st = device_layer(s, ch, p, keeps[i])) != PP_OK) return st;
+                if (++next[i] == run_end[i] && (st = plan.finish_run(s)) != PP_OK) return st;
+                continue;
+            }
+            idx.push_back(i);
+            chans.push_back(ch);
+            preps.push_back(p);
+            ks.push_back(keeps[i]);
+        }
+        if (idx.empty()) {
+            bool left = false;
+            for (uint32_t i = 0; i < count; ++i) left = left || next[i] < plans[i].n_channels;
+            if (!left) break;
+            continue;  // every instance took a key-preserving layer this round
+        }
+        st = b->W == 1 ? (b->kind == PP_COEFF_F64 ? batch_step<1, double>(b, idx, chans, preps, ks)
+                                                 : batch_step<1, float>(b, idx, chans, preps, ks))
+                       : (b->kind == PP_COEFF_F64 ? batch_step<2, double>(b, idx, chans, preps, ks)
+                                                 : batch_step<2, float>(b, idx, chans, preps, ks));
+        if (st != PP_OK) return st;
+        for (uint32_t i : idx)
+            if (++next[i] == run_end[i] && (st = plans[i].finish_run(b->sums[i])) != PP_OK) return st;
+    }
+    return PP_OK;
+}
 
 // ---- tuning -------------------------------------------------------------------------------------------------------
 

@@ -149,6 +149,13 @@ constexpr uint32_t kJoinOnePairPerGroup = 0xFFFFFFFFu;
 // block: 256 or 512 threads per workgroup (a chunk of that many rows per pass).
 template <int W, typename T>
 hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t groups_per_cu, uint32_t block);
+// One layer for each of num_items sums (device arrays: items, pair_start = exclusive prefix of their pair counts,
+// total_pairs in all): their output offsets, the joins (each item staging its largest pair, scap = lcap), and
+// their counters, also gathered into out[4 i + 0..2] = rows, longest bucket, overflow flag.
+template <int W, typename T>
+hipError_t launch_join_batch(hipStream_t st, const JoinArgs<W, T>* items, const uint32_t* pair_start,
+                             uint32_t num_items, uint32_t total_pairs, uint32_t scap_max, uint32_t tcap_bits_max,
+                             unsigned long long* out);
 template <int W, typename T>
 hipError_t launch_rescale(hipStream_t st, const RescaleArgs<W, T>& a, uint32_t num_buckets);
 template <int W, typename T>

@@ -349,10 +349,11 @@ __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const 
 
 // REDO: the launch over the queued pairs (a compile-time switch, so that the first launch's pair index needs no
 // load: a lookup there, merged with the prefetch's path, would make the compiler wait for the prefetch).
+// The join of work items t, t + G, t + G + G, ... below count (see k_rotation_join); lds and scan_scratch are the
+// workgroup's.
 template <int W, typename T, bool REDO, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ uint32_t scan_scratch[4 * (BLOCK / 64)];
+__device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, uint32_t G, uint32_t count,
+                                           unsigned char* lds, uint32_t* scan_scratch) {
     const uint32_t scap = a.scap;  // staged rows this launch holds
     uint64_t* const Lx = reinterpret_cast<uint64_t*>(lds);   // [W][scap]
     uint64_t* const Lz = Lx + (size_t)W * scap;              // [W][scap]
@@ -364,9 +365,6 @@ __global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a)
 
     // Work item t is pair t, or in the redo launch the t-th pair the first launch queued.
     constexpr bool redo = REDO;
-    const uint32_t count = redo ? (uint32_t)*a.redo_count : a.num_cosets;
-    const uint32_t G = gridDim.x;
-    uint32_t t = blockIdx.x;
     if (t >= count) return;  // uniform
     const uint32_t last = count - 1u;
     auto pair_at = [&](uint32_t i) { return redo ? a.redo[i] : i; };
@@ -460,7 +458,7 @@ __global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a)
                 // Queue the pair for the redo launch, which stages up to the largest pair; in the redo launch
                 // itself an overflow means the host's bound was wrong.
                 if (threadIdx.x == 0) {
-                    if (redo) {
+                    if (redo || a.redo == nullptr) {
                         atomicOr(&a.stats[2], 1ull);
                     } else {
                         a.redo[atomicAdd(a.redo_count, 1ull)] = p;
@@ -500,6 +498,95 @@ __global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a)
         }
         base = nbase;
         cb = nb;
+    }
+}
+
+template <int W, typename T, bool REDO, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t scan_scratch[4 * (BLOCK / 64)];
+    const uint32_t count = REDO ? (uint32_t)*a.redo_count : a.num_cosets;
+    join_items<W, T, REDO, BLOCK>(a, blockIdx.x, gridDim.x, count, lds, scan_scratch);
+}
+
+// Many sums' layers in one launch: workgroup w joins pair w - pair_start[i] of item i, where item i is the one
+// whose range [pair_start[i], pair_start[i + 1]) holds w.  Every item stages its largest pair (no redo).
+template <int W, typename T, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_rotation_join_batch(const JoinArgs<W, T>* __restrict__ items,
+                                                              const uint32_t* __restrict__ pair_start,
+                                                              uint32_t num_items) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t scan_scratch[4 * (BLOCK / 64)];
+    uint32_t lo = 0, hi = num_items;  // the last item whose start is at most blockIdx.x
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pair_start[mid] <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const JoinArgs<W, T>& a = items[lo];
+    const uint32_t t = blockIdx.x - pair_start[lo];
+    join_items<W, T, false, BLOCK>(a, t, a.num_cosets, a.num_cosets, lds, scan_scratch);
+}
+
+// The batch's bucket offsets and counters, one workgroup per item: the output segments of a join (as
+// k_offsets_* mode 1: len[b] + len[b ^ delta], or 2 len[b], rounded to kSegAlign) and, after the join, the total
+// and longest bucket of its output.  Small sums are the batch's use, so one workgroup scans an item's buckets.
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_offsets_batch(const JoinArgs<W, T>* __restrict__ items) {
+    __shared__ uint64_t wave_sums[kWaves];
+    const JoinArgs<W, T>& a = items[blockIdx.x];
+    const uint32_t B = a.delta ? 2u * a.num_cosets : a.num_cosets;
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < B; base += kBlock) {
+        const uint32_t b = base + threadIdx.x;
+        uint64_t v = 0;
+        if (b < B) {
+            v = a.delta ? (uint64_t)a.len[b] + a.len[b ^ a.delta] : 2ull * a.len[b];
+            v = (v + (kSegAlign - 1)) & ~(uint64_t)(kSegAlign - 1);
+        }
+        uint64_t total;
+        const uint64_t ex = block_exclusive_scan_u64(v, wave_sums, &total);
+        if (b < B) const_cast<uint64_t*>(a.ostart)[b] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x < 8) a.stats[threadIdx.x] = 0;
+}
+
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_reduce_batch(const JoinArgs<W, T>* __restrict__ items,
+                                                        unsigned long long* __restrict__ out) {
+    __shared__ uint64_t wave_sums[kWaves];
+    __shared__ uint32_t wave_max[kWaves];
+    const JoinArgs<W, T>& a = items[blockIdx.x];
+    const uint32_t B = a.delta ? 2u * a.num_cosets : a.num_cosets;
+    uint64_t s = 0;
+    uint32_t m = 0;
+    for (uint32_t b = threadIdx.x; b < B; b += kBlock) {
+        const uint32_t v = a.olen[b];
+        s += v;
+        m = max(m, v);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        s += __shfl_xor(s, d, 64);
+        m = max(m, (uint32_t)__shfl_xor((int)m, d, 64));
+    }
+    if (lane_id() == 0) {
+        wave_sums[wave_id()] = s;
+        wave_max[wave_id()] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t ts = 0;
+        uint32_t tm = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            ts += wave_sums[w];
+            tm = max(tm, wave_max[w]);
+        }
+        a.stats[0] = ts;
+        a.stats[1] = tm;
+        out[4 * blockIdx.x + 0] = ts;
+        out[4 * blockIdx.x + 1] = tm;
+        out[4 * blockIdx.x + 2] = a.stats[2];
     }
 }
 
@@ -755,6 +842,21 @@ hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_
 }
 
 template <int W, typename T>
+hipError_t launch_join_batch(hipStream_t st, const JoinArgs<W, T>* items, const uint32_t* pair_start,
+                             uint32_t num_items, uint32_t total_pairs, uint32_t scap_max, uint32_t tcap_bits_max,
+                             unsigned long long* out) {
+    if (num_items == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_offsets_batch<W, T>), dim3(num_items), dim3(kBlock), 0, st, items);
+    if (total_pairs) {
+        const size_t lds = join_lds_bytes<W, T>(scap_max, tcap_bits_max);
+        hipLaunchKernelGGL((k_rotation_join_batch<W, T, 256>), dim3(total_pairs), dim3(256), lds, st, items,
+                           pair_start, num_items);
+    }
+    hipLaunchKernelGGL((k_reduce_batch<W, T>), dim3(num_items), dim3(kBlock), 0, st, items, out);
+    return hipGetLastError();
+}
+
+template <int W, typename T>
 hipError_t launch_rescale(hipStream_t st, const RescaleArgs<W, T>& a, uint32_t num_buckets) {
     if (num_buckets == 0) return hipSuccess;
     hipLaunchKernelGGL((k_rescale<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a);
@@ -791,6 +893,8 @@ hipError_t launch_export(hipStream_t st, const ExportArgs<W, T>& a, uint32_t num
 #define PP_INSTANTIATE(W, T)                                                                           \
     template size_t join_lds_bytes<W, T>(uint32_t, uint32_t);                                          \
     template hipError_t launch_rotation_join<W, T>(hipStream_t, const JoinArgs<W, T>&, uint32_t, uint32_t); \
+    template hipError_t launch_join_batch<W, T>(hipStream_t, const JoinArgs<W, T>*, const uint32_t*, uint32_t,     \
+                                                uint32_t, uint32_t, uint32_t, unsigned long long*);             \
     template hipError_t launch_rescale<W, T>(hipStream_t, const RescaleArgs<W, T>&, uint32_t);          \
     template hipError_t launch_refine<W, T>(hipStream_t, const RefineArgs<W, T>&);                      \
     template hipError_t launch_coarsen<W, T>(hipStream_t, const CoarsenArgs<W, T>&);                    \

@@ -23,6 +23,7 @@ from . import _native, circuits, gates, truncation
 from .circuits import Circuit
 
 __all__ = ["PauliSum", "GpuPauliSum", "Circuit", "PropagationStats", "propagated", "propagated_with_stats",
+           "propagate_batch",
            "truncation", "gates", "circuits", "DEFAULT_HASH_SEED", "device_available"]
 
 #: pauli_sum/storage.rs:12
@@ -396,3 +397,38 @@ def propagated_with_stats(initial: PauliSum, circuit: Circuit, policy=None, dire
     out = initial.copy()
     stats = out.propagate_with_stats(circuit, policy, direction, **kw)
     return out, stats
+
+
+def propagate_batch(sums, circuits, policies=None, directions=None, target_bucket_len: int = 1024,
+                    min_buckets: int = 128) -> None:
+    """Propagate many resident sums (``GpuPauliSum``, one context, one width and coefficient type) each through its
+    own circuit, stepped together: one launch of each batched kernel per layer step (pp_batch_propagate).  Every
+    sum ends exactly as ``propagate`` would leave it."""
+    sums = list(sums)
+    circuits = list(circuits)
+    if len(circuits) != len(sums):
+        raise ValueError("one circuit per sum")
+    if not sums:
+        return
+    ctx = sums[0].ctx
+    policies = list(policies) if policies is not None else [None] * len(sums)
+    directions = list(directions) if directions is not None else [None] * len(sums)
+    chans, offsets = [], [0]
+    for c in circuits:
+        ch = c.channels() if isinstance(c, Circuit) else np.asarray(c)
+        chans.extend(ch)
+        offsets.append(offsets[-1] + len(ch))
+    arr = (_native.Channel * max(1, len(chans)))()
+    for i, row in enumerate(chans):
+        arr[i] = GpuPauliSum._channel(row)
+    offs = (ctypes.c_uint64 * len(offsets))(*offsets)
+    dirs = (ctypes.c_int * len(sums))(*[_parse_direction(d) for d in directions])
+    pols = (_native.Policy * len(sums))(*[truncation.lower(p) for p in policies])
+    handles = (ctypes.c_void_p * len(sums))(*[s._h.value for s in sums])
+    b = ctypes.c_void_p()
+    _native.check(_native.lib().pp_batch_create(ctx.handle, handles, len(sums), ctypes.byref(b)), ctx.handle)
+    try:
+        opts = _options(target_bucket_len, min_buckets)
+        _native.check(_native.lib().pp_batch_propagate(b, arr, offs, dirs, pols, ctypes.byref(opts)), ctx.handle)
+    finally:
+        _native.lib().pp_batch_free(b)

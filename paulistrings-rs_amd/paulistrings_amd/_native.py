@@ -99,6 +99,10 @@ def lib() -> ctypes.CDLL:
     L.pp_apply_rotation_to.argtypes = [vp, vp, ctypes.POINTER(Channel), ctypes.c_int, ctypes.POINTER(Policy)]
     L.pp_truncate_topn.argtypes = [vp, u64]
     L.pp_truncate_approx_topn.argtypes = [vp, u64]
+    L.pp_batch_create.argtypes = [vp, ctypes.POINTER(vp), u32, pvp]
+    L.pp_batch_free.argtypes = [vp]
+    L.pp_batch_propagate.argtypes = [vp, ctypes.POINTER(Channel), ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(Policy), ctypes.POINTER(Options)]
     L.pp_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.pp_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.pp_autotune.argtypes = [vp, ctypes.POINTER(Channel), u64, ctypes.POINTER(Policy), ctypes.POINTER(Tuning)]
@@ -116,7 +120,8 @@ def lib() -> ctypes.CDLL:
                  "pp_sum_free", "pp_sum_len", "pp_sum_download", "pp_sum_bucket_bits", "pp_propagate",
                  "pp_apply_rotation", "pp_apply_rotation_to", "pp_truncate_topn", "pp_truncate_approx_topn",
                  "pp_graph_capture_begin", "pp_graph_capture_end", "pp_graph_launch", "pp_graph_free",
-                 "pp_get_tuning", "pp_set_tuning", "pp_autotune",
+                 "pp_get_tuning", "pp_set_tuning", "pp_autotune", "pp_batch_create", "pp_batch_free",
+                 "pp_batch_propagate",
                  "pp_expectation_product", "pp_profile_enable", "pp_profile_read"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L

@@ -220,3 +220,44 @@ def repeat(step: Circuit, times: int) -> Circuit:
     for _ in range(times):
         out.extend(step)
     return out
+
+
+def batched_instances(seed: int = 0xC0FFEE, count: int = 64, depths: tuple[int, int] = (5, 30),
+                      thresholds: tuple[float, float] = (1e-6, 1e-3)):
+    """The batched workload: ``count`` independent propagations with qubit counts uniform in {16..64}, depths uniform
+    in ``depths`` (inclusive) brick layers (RZZ on the even then the odd bonds of a chain, alternating by layer, then
+    RX or RZ by a coin flip on every qubit, angles uniform in [0, 2 pi)), observables of 1..8 random Paulis of weight
+    at most 3 with N(0, 1) coefficients, and coefficient thresholds log-uniform in ``thresholds``.  Drawn with numpy's
+    default generator from ``seed``.  Returns a list of (num_qubits, circuit, (x, z, coeff), threshold)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    lo_eps, hi_eps = math.log10(thresholds[0]), math.log10(thresholds[1])
+    for _ in range(count):
+        n = int(rng.integers(16, 65))
+        depth = int(rng.integers(depths[0], depths[1] + 1))
+        c = Circuit(n)
+        for layer in range(depth):
+            for i in range(layer % 2, n - 1, 2):
+                c.rzz(float(rng.uniform(0.0, 2 * math.pi)), i, i + 1)
+            for q in range(n):
+                theta = float(rng.uniform(0.0, 2 * math.pi))
+                if rng.random() < 0.5:
+                    c.rx(theta, q)
+                else:
+                    c.rz(theta, q)
+        terms = int(rng.integers(1, 9))
+        W = width_for(n)
+        x = np.zeros((terms, W), np.uint64)
+        z = np.zeros((terms, W), np.uint64)
+        for t in range(terms):
+            k = int(rng.integers(1, 4))
+            for q in rng.choice(n, size=k, replace=False):
+                p = int(rng.integers(1, 4))  # X, Y, Z
+                if p in (1, 2):
+                    x[t, q // 64] |= np.uint64(1) << np.uint64(q % 64)
+                if p in (2, 3):
+                    z[t, q // 64] |= np.uint64(1) << np.uint64(q % 64)
+        coeff = rng.standard_normal(terms)
+        eps = float(10.0 ** rng.uniform(lo_eps, hi_eps))
+        out.append((n, c, (x, z, coeff), eps))
+    return out

@@ -1,0 +1,81 @@
+"""Batched propagation (pp_batch_propagate): many sums stepped together, one launch of each batched kernel per
+layer step, each sum ending bitwise as its solo oracle run ends.  The workload is the batched recipe (qubits
+16..64, brick-wall RZZ + RX/RZ circuits, weight <= 3 observables) at depths and thresholds that the oracle finishes
+in seconds; a few instances carry a TopN or ApproxTopN pass or a key-preserving channel.  Each case prints its
+name before it touches the GPU."""
+
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import pp_oracle as po
+from paulistrings_amd import Circuit, PauliSum, circuits, propagate_batch, truncation
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg: str) -> None:
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def check_batch(instances, policies_o, policies_p, direction="heisenberg"):
+    gpus = []
+    for (n, circ, (x, z, c), _eps) in instances:
+        gpus.append(PauliSum(n, x, z, c).to_device(0))
+    propagate_batch(gpus, [inst[1] for inst in instances], policies_p, [direction] * len(instances))
+    for k, ((n, circ, (x, z, c), _eps), g) in enumerate(zip(instances, gpus)):
+        s = po.OracleSum(n, x, z, c)
+        s.propagate(circ.channels(), direction, policies_o[k])
+        ox, oz, oc, _ = s.export()
+        h = g.to_host()
+        g.free()
+        assert len(h.coefficients_array()) == len(oc), (k, len(h.coefficients_array()), len(oc))
+        assert np.array_equal(h.x_array(), ox) and np.array_equal(h.z_array(), oz), k
+        assert np.array_equal(h.coefficients_array(), oc), k
+
+
+def test_batched_instances_equal_their_solo_runs():
+    announce("batch of 24 brick-wall instances")
+    inst = circuits.batched_instances(seed=7, count=24, depths=(3, 8), thresholds=(3e-4, 1e-2))
+    # a key-preserving layer (RX(pi): its sine snaps to 0) in two of them
+    for k in (3, 11):
+        n, circ, obs, eps = inst[k]
+        circ.rx(math.pi, 0)
+    pol_o, pol_p = [], []
+    for k, (_n, _c, _o, eps) in enumerate(inst):
+        if k == 5:
+            pol_o.append(po.coeff_policy(eps=eps, topn=300))
+            pol_p.append(truncation.coeff(eps) & truncation.topn(300))
+        elif k == 6:
+            pol_o.append(po.coeff_policy(eps=eps, approx_topn=250))
+            pol_p.append(truncation.coeff(eps) & truncation.approx_topn(250))
+        else:
+            pol_o.append(po.coeff_policy(eps=eps))
+            pol_p.append(truncation.coeff(eps))
+    check_batch(inst, pol_o, pol_p)
+
+
+def test_batch_of_wide_sums_forward():
+    announce("batch of 3 sums on 100-127 qubits, forward")
+    rng = np.random.default_rng(12)
+    inst = []
+    for n in (100, 127, 113):
+        m = 20000
+        x = np.zeros((m, 2), np.uint64)
+        z = np.zeros((m, 2), np.uint64)
+        for w in range(2):
+            live = min(64, n - 64 * w)
+            weights = np.uint64(1) << np.arange(live, dtype=np.uint64)
+            x[:, w] = ((rng.random((m, live)) < 0.1) * weights).sum(axis=1, dtype=np.uint64)
+            z[:, w] = ((rng.random((m, live)) < 0.1) * weights).sum(axis=1, dtype=np.uint64)
+        c = Circuit(n)
+        for _ in range(6):
+            q = int(rng.integers(0, n - 1))
+            c.rzz(float(rng.uniform(0, 2 * math.pi)), q, q + 1).rx(float(rng.uniform(0, 2 * math.pi)), int(rng.integers(0, n)))
+        inst.append((n, c, (x, z, rng.standard_normal(m)), 1e-4))
+    check_batch(inst, [po.coeff_policy(eps=1e-4)] * 3, [truncation.coeff(1e-4)] * 3, direction="forward")

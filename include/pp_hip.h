@@ -172,6 +172,39 @@ pp_status pp_batch_free(pp_batch* batch);
 pp_status pp_batch_propagate(pp_batch* batch, const pp_channel* channels, const uint64_t* offsets,
                              const pp_direction* directions, const pp_policy* policies, const pp_options* options);
 
+/* ---- the learned truncation policy --------------------------------------------------------------- */
+
+/* No counterpart in the reference: a TruncationPolicy (truncation/mod.rs:30-113) whose layer pass keeps the n
+ * terms of largest predicted contribution to the final expectation value, with plain |c| TopN
+ * (truncation/builtin.rs:61-150) as its fallback and judge.  The predictor is a 35-128-128-1 ReLU MLP over a
+ * one-word term's features: X and Z occupancy counts of 16 qubit groups (a lattice's rows, then its columns;
+ * unused groups 0), its Pauli weight, ln|c| and the layers still to apply (divided by depth_scale).  Its two hidden
+ * GEMMs run in bf16 on MFMA with fp32 accumulation; parameters and Adam state are fp32 on the device.
+ * Parameters: pp_model_param_count() floats, W1 [64][128] (input-major, rows 35..63 zero), b1 [128],
+ * W2 [128][128], b2 [128], w3 [128], b3.  Samples are one-word terms: x, z, f64 coefficient, remaining layers. */
+typedef struct pp_model pp_model;
+uint32_t pp_model_param_count(void);
+pp_status pp_model_create(pp_context* ctx, const uint64_t* groups, float depth_scale, uint64_t seed,
+                          pp_model** out);
+pp_status pp_model_free(pp_model* model);
+pp_status pp_model_set_params(pp_model* model, const float* params);  /* also resets the optimizer */
+pp_status pp_model_get_params(pp_model* model, float* params);
+pp_status pp_model_predict(pp_model* model, const uint64_t* x, const uint64_t* z, const double* coeff,
+                           const float* remaining, uint64_t n, float* out);
+/* Gradient of the mean squared error of the predictions against labels, and that error. */
+pp_status pp_model_gradient(pp_model* model, const uint64_t* x, const uint64_t* z, const double* coeff,
+                            const float* remaining, const float* labels, uint64_t n, float* grads, double* loss);
+/* Adam (0.9, 0.999, 1e-8) over `epochs` shuffled passes in batches of `batch`; loss = the last epoch's mean. */
+pp_status pp_model_train(pp_model* model, const uint64_t* x, const uint64_t* z, const double* coeff,
+                         const float* remaining, const float* labels, uint64_t n, uint32_t epochs, uint32_t batch,
+                         float lr, uint64_t seed, double* loss);
+/* The learned layer pass on a one-word sum: keep the n terms of largest score (ties in canonical order). */
+pp_status pp_truncate_learned(pp_sum* sum, pp_model* model, uint64_t n, float remaining);
+/* pp_propagate with the learned pass (keep n) after every run; `filter` is a per-term policy without a pass. */
+pp_status pp_propagate_learned(pp_sum* sum, const pp_channel* channels, uint64_t n_channels, pp_direction direction,
+                               const pp_policy* filter, pp_model* model, uint64_t keep, const pp_options* options,
+                               uint64_t* layer_len);
+
 /* ---- tuning ------------------------------------------------------------------------------------- */
 
 /* The device path's sizes, per context.  No counterpart in the reference, which fixes its device sizes

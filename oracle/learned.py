@@ -124,8 +124,9 @@ def loss(p: Params, f: np.ndarray, t: np.ndarray, exact: bool = False) -> float:
 
 
 def backward(p: Params, f: np.ndarray, t: np.ndarray, exact: bool = False) -> Params:
-    """Gradients of the mean squared error; the bf16 rounding is treated as the identity (straight-through), which is
-    what the device's backward computes: dW2 = a1^T dz2 with a1 the bf16 hidden activations, dW1 = a0^T dz1."""
+    """Gradients of the mean squared error, the rounding treated as the identity (straight-through), with the
+    device's GEMM operands: dW2 = a1^T bf16(dz2), dh1 = bf16(dz2) bf16(W2)^T, dW1 = a0^T bf16(dz1) (a0, a1 the
+    bf16 inputs and hidden activations); bias and last-layer gradients from the fp32 values."""
     y, (a0, z1, a1, z2, h2) = forward(p, f, exact)
     rnd = (lambda a: np.asarray(a, np.float32)) if exact else bf16_round
     m = len(t)
@@ -134,12 +135,13 @@ def backward(p: Params, f: np.ndarray, t: np.ndarray, exact: bool = False) -> Pa
     g.w3 = (h2.astype(np.float64).T @ dy).astype(np.float32)
     g.b3 = np.float32(dy.sum())
     dh2 = np.outer(dy, p.w3.astype(np.float64))
-    dz2 = dh2 * (z2 > 0)
-    g.W2 = (a1.T @ dz2).astype(np.float32)
-    g.b2 = dz2.sum(axis=0).astype(np.float32)
-    dh1 = dz2 @ rnd(p.W2).astype(np.float64).T
-    dz1 = dh1 * (z1 > 0)
-    g.W1 = (a0.T @ dz1).astype(np.float32)
+    dz2 = (dh2 * (z2 > 0)).astype(np.float32)
+    dz2r = rnd(dz2).astype(np.float64)  # the GEMMs' operand
+    g.W2 = (a1.T @ dz2r).astype(np.float32)
+    g.b2 = dz2.astype(np.float64).sum(axis=0).astype(np.float32)
+    dh1 = dz2r @ rnd(p.W2).astype(np.float64).T
+    dz1 = (dh1 * (z1 > 0)).astype(np.float32)
+    g.W1 = (a0.T @ rnd(dz1).astype(np.float64)).astype(np.float32)
     g.W1[IN:] = 0.0
-    g.b1 = dz1.sum(axis=0).astype(np.float32)
+    g.b1 = dz1.astype(np.float64).sum(axis=0).astype(np.float32)
     return g

@@ -17,6 +17,7 @@
 
 #include "../../include/pp_hip.h"
 #include "pp_kernels.h"
+#include "pp_learned.h"
 #include "pp_topn.h"
 
 namespace {
@@ -233,6 +234,8 @@ struct pp_sum {
     unsigned long long* d_hist = nullptr;
     uint32_t* d_redo = nullptr;  // pairs a join could not stage, for its redo launch
     uint64_t redo_cap = 0;
+    float* d_score = nullptr;  // the learned pass's score per column position
+    uint64_t score_cap = 0;
     size_t coeff_bytes() const { return kind == PP_COEFF_F64 ? 8 : 4; }
 };
 
@@ -714,7 +717,7 @@ pp_status apply_one(pp_sum* s, const pp_channel& ch, const Prepared& p, const Ke
     return device_layer(s, ch, p, keep);
 }
 
-pp_status topn_device(pp_sum* s, uint64_t n);         // pp_topn_impl.inc
+pp_status topn_device(pp_sum* s, uint64_t n, const float* score = nullptr);  // pp_topn_impl.inc
 pp_status approx_topn_device(pp_sum* s, uint64_t n);  // pp_topn_impl.inc
 
 pp_status check_channels(pp_sum* s, const pp_channel* channels, uint64_t n_channels) {
@@ -963,6 +966,7 @@ pp_status pp_sum_free(pp_sum* s) {
     if (s->h_stats) (void)hipHostFree(s->h_stats);
     if (s->d_topn) (void)hipFree(s->d_topn);
     if (s->d_redo) (void)hipFree(s->d_redo);
+    if (s->d_score) (void)hipFree(s->d_score);
     if (s->d_hist) (void)hipFree(s->d_hist);
     if (s->d_tiles) (void)hipFree(s->d_tiles);
     delete s;
@@ -1667,5 +1671,7 @@ pp_status pp_graph_free(pp_graph* g) {
     return PP_OK;
 }
 
-// The TopN pass lives in pp_topn.hip; it needs the sum's internals.
+// The TopN pass lives in pp_topn.hip and the learned network in pp_learned.hip; their host halves need the sum's
+// internals.
 #include "pp_topn_impl.inc"
+#include "pp_learned_impl.inc"

@@ -31,6 +31,7 @@ __host__ __device__ inline int topn_bits(int pass) { return pass < 5 ? 11 : 9; }
 template <typename T>
 struct TopnHistArgs {
     const T* c;
+    const float* score;  // when set, the ranking key is this score (largest kept) instead of |c|^2
     const uint64_t* start;
     const uint32_t* len;
     uint32_t num_buckets;
@@ -45,6 +46,7 @@ struct TopnTieArgs {
     const uint64_t* x;
     const uint64_t* z;
     const T* c;
+    const float* score;  // when set, the ranking key is this score (largest kept) instead of |c|^2
     uint64_t cap;
     const uint64_t* start;
     const uint32_t* len;
@@ -63,6 +65,7 @@ struct TopnRetainArgs {
     uint64_t* x;
     uint64_t* z;
     T* c;
+    const float* score;  // when set, the ranking key is this score (largest kept) instead of |c|^2
     uint64_t cap;
     const uint64_t* start;
     uint32_t* len;

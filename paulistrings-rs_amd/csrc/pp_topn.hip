@@ -16,6 +16,20 @@ __device__ __forceinline__ uint64_t magnitude_bits(T c) {
     return (uint64_t)__double_as_longlong(v * v);
 }
 
+// A float score as a key in the high half of a u64 that orders as the score (a NaN score sorts lowest): the learned
+// TopN pass ranks terms by their score through the same selection as TopN.
+__device__ __forceinline__ uint64_t score_bits(float s) {
+    if (s != s) return 0ull;
+    const uint32_t u = __float_as_uint(s);
+    const uint32_t k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return (uint64_t)k << 32;
+}
+
+template <typename T>
+__device__ __forceinline__ uint64_t rank_bits(const T* c, const float* score, uint64_t i) {
+    return score ? score_bits(score[i]) : magnitude_bits<T>(c[i]);
+}
+
 __global__ void k_topn_init(TopnState* s, uint64_t n) {
     s->prefix = 0;
     s->mask = 0;
@@ -42,8 +56,8 @@ __global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
             const uint32_t n = a.len[b];
             const uint64_t s = a.start[b];
             for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
-                const uint64_t u = magnitude_bits<T>(a.c[s + r]);
-                if (!is_nan_bits(u) && (u & st.mask) == st.prefix) atomicAdd(&h[(u >> shift) & dmask], 1u);
+                const uint64_t u = rank_bits<T>(a.c, a.score, s + r);
+                if ((a.score || !is_nan_bits(u)) && (u & st.mask) == st.prefix) atomicAdd(&h[(u >> shift) & dmask], 1u);
             }
         }
     }
@@ -92,7 +106,7 @@ __global__ void __launch_bounds__(kBlock) k_topn_gather_ties(const TopnTieArgs<W
     const uint64_t s = a.start[b];
     const uint64_t t = a.state->prefix;
     for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
-        const uint64_t u = magnitude_bits<T>(a.c[s + r]);
+        const uint64_t u = rank_bits<T>(a.c, a.score, s + r);
         if (u != t) continue;
         const uint64_t slot = atomicAdd(&a.state->gathered, 1ull);
         if (slot >= a.tie_cap) continue;
@@ -123,9 +137,10 @@ __global__ void __launch_bounds__(kBlock) k_topn_retain(const TopnRetainArgs<W, 
         if (r < n) {
             k = load_key<W>(a.x, a.z, a.cap, s + r);
             c = a.c[s + r];
-            const uint64_t u = magnitude_bits<T>(c);
-            if (st.keep_all_finite) keep = !is_nan_bits(u);
-            else if (is_nan_bits(u)) keep = false;
+            const uint64_t u = rank_bits<T>(a.c, a.score, s + r);
+            const bool nan = !a.score && is_nan_bits(u);
+            if (st.keep_all_finite) keep = !nan;
+            else if (nan) keep = false;
             else if (u > st.prefix) keep = true;
             else if (u == st.prefix) keep = a.tie_keep == nullptr || a.tie_keep[s + r] != 0;
         }

@@ -178,8 +178,10 @@ pp_status pp_batch_propagate(pp_batch* batch, const pp_channel* channels, const 
  * terms of largest predicted contribution to the final expectation value, with plain |c| TopN
  * (truncation/builtin.rs:61-150) as its fallback and judge.  The predictor is a 35-128-128-1 ReLU MLP over a
  * one-word term's features: X and Z occupancy counts of 16 qubit groups (a lattice's rows, then its columns;
- * unused groups 0), its Pauli weight, ln|c| and the layers still to apply (divided by depth_scale).  Its two hidden
- * GEMMs run in bf16 on MFMA with fp32 accumulation; parameters and Adam state are fp32 on the device.
+ * unused groups 0), its Pauli weight, ln|c| and the layers still to apply (divided by depth_scale, capped at 1).
+ * It is trained to output ln|<psi| P propagated through the remaining layers |psi>|, the term's read-out factor;
+ * the pass ranks by ln|c| plus that output.  Its two hidden GEMMs run in bf16 on MFMA with fp32 accumulation;
+ * parameters and Adam state are fp32 on the device.
  * Parameters: pp_model_param_count() floats, W1 [64][128] (input-major, rows 35..63 zero), b1 [128],
  * W2 [128][128], b2 [128], w3 [128], b3.  Samples are one-word terms: x, z, f64 coefficient, remaining layers. */
 typedef struct pp_model pp_model;
@@ -198,7 +200,8 @@ pp_status pp_model_gradient(pp_model* model, const uint64_t* x, const uint64_t* 
 pp_status pp_model_train(pp_model* model, const uint64_t* x, const uint64_t* z, const double* coeff,
                          const float* remaining, const float* labels, uint64_t n, uint32_t epochs, uint32_t batch,
                          float lr, uint64_t seed, double* loss);
-/* The learned layer pass on a one-word sum: keep the n terms of largest score (ties in canonical order). */
+/* The learned layer pass on a one-word sum: keep the n terms of largest score ln|c| + prediction (ties in canonical
+ * order), the prediction being the network's output for the term's features with `remaining` layers left. */
 pp_status pp_truncate_learned(pp_sum* sum, pp_model* model, uint64_t n, float remaining);
 /* pp_propagate with the learned pass (keep n) after every run; `filter` is a per-term policy without a pass. */
 pp_status pp_propagate_learned(pp_sum* sum, const pp_channel* channels, uint64_t n_channels, pp_direction direction,

@@ -15,7 +15,7 @@ Features of a term (x, z words, coefficient c) on a lattice of up to 16 row/colu
   [16, 32)  popcount(z & group_g) / 8      (Z or Y)
   32        Pauli weight / 16
   33        ln|c| / 8
-  34        remaining layers / depth scale
+  34        min(remaining layers / depth scale, 1): the depth scale is the horizon past which depths read alike
   [35, 64)  zero padding (the first GEMM's K is 64)
 """
 
@@ -69,7 +69,7 @@ def features(x: np.ndarray, z: np.ndarray, c: np.ndarray, groups: np.ndarray, re
         f[:, 16 + g] = popcount64(z & groups[g]) / 8.0
     f[:, 32] = popcount64(x | z) / 16.0
     f[:, 33] = (np.log(np.abs(np.asarray(c, np.float64))) / 8.0).astype(np.float32)
-    f[:, 34] = np.float32(remaining / depth_scale)
+    f[:, 34] = np.minimum(np.float32(remaining / depth_scale), np.float32(1.0))
     return f
 
 

@@ -209,6 +209,8 @@ struct BucketArrays {
     uint64_t cap = 0;
 };
 
+enum ScratchSlot { kScrKeyX, kScrKeyZ, kScrPos, kScrBucket, kScrFlags, kScrCounts, kScrGathered, kScratchSlots };
+
 struct pp_sum {
     pp_context* ctx = nullptr;
     uint32_t num_qubits = 0;
@@ -236,6 +238,10 @@ struct pp_sum {
     uint64_t redo_cap = 0;
     float* d_score = nullptr;  // the learned pass's score per column position
     uint64_t score_cap = 0;
+    // grow-only scratch of the truncation passes (boundary terms, their flags and counts), kept between layers so
+    // that a pass run after every layer allocates nothing once the sizes have settled
+    void* scratch[kScratchSlots] = {};
+    size_t scratch_cap[kScratchSlots] = {};
     size_t coeff_bytes() const { return kind == PP_COEFF_F64 ? 8 : 4; }
 };
 
@@ -366,6 +372,22 @@ pp_status ensure_tiles(pp_sum* s, uint32_t out_n) {
     pp_status st = dev_alloc(s->ctx, (void**)&s->d_tiles, need * 8);
     if (st != PP_OK) return st;
     s->tiles_cap = need;
+    return PP_OK;
+}
+
+// Slot `slot` of the sum's truncation scratch with room for `bytes` (grown to 1.5x the request when it must grow).
+template <typename P>
+pp_status scratch(pp_sum* s, ScratchSlot slot, size_t bytes, P** out) {
+    if (s->scratch_cap[slot] < bytes || !s->scratch[slot]) {
+        if (s->scratch[slot]) (void)hipFree(s->scratch[slot]);
+        s->scratch[slot] = nullptr;
+        s->scratch_cap[slot] = 0;
+        const size_t grown = std::max<size_t>(256, bytes + bytes / 2);
+        pp_status st = dev_alloc(s->ctx, &s->scratch[slot], grown);
+        if (st != PP_OK) return st;
+        s->scratch_cap[slot] = grown;
+    }
+    *out = (P*)s->scratch[slot];
     return PP_OK;
 }
 
@@ -969,6 +991,8 @@ pp_status pp_sum_free(pp_sum* s) {
     if (s->d_score) (void)hipFree(s->d_score);
     if (s->d_hist) (void)hipFree(s->d_hist);
     if (s->d_tiles) (void)hipFree(s->d_tiles);
+    for (void* p : s->scratch)
+        if (p) (void)hipFree(p);
     delete s;
     return PP_OK;
 }

@@ -25,7 +25,7 @@ struct MlpDev {
     const uint16_t* w2t;   // bf16 W2 transposed: [128 outputs][128 inputs]
     const uint16_t* w2;    // bf16 W2 input-major: [128 inputs][128 outputs] (the backward's operand)
     uint64_t groups[16];   // qubit groups (lattice rows, then columns) whose X and Z occupancies are features
-    float depth_scale;     // remaining layers are divided by it
+    float depth_scale;     // remaining layers are divided by it, the quotient capped at 1 (the horizon)
 };
 
 // Samples of one word: keys, coefficients (f64), remaining layers, and (training) labels.
@@ -45,8 +45,9 @@ hipError_t launch_mlp_grad(hipStream_t st, const MlpDev& m, const MlpBatch& b, f
 hipError_t launch_mlp_adam(hipStream_t st, float* params, float* adam_m, float* adam_v, const float* grads,
                            float lr, float beta1, float beta2, float eps, int step);
 
-// The score of every term of a resident one-word sum (the predicted ln|contribution|), written at the term's column
-// position: score[start[b] + r] for r < len[b].
+// The score of every term of a resident one-word sum, ln|c| + the network's output (the predicted ln of the term's
+// read-out factor |<psi| P propagated |psi>|), so the predicted ln|contribution|; written at the term's column
+// position: score[start[b] + r] for r < len[b].  A network that outputs a constant ranks exactly as |c| does.
 template <typename T>
 struct ScoreArgs {
     const uint64_t* x;

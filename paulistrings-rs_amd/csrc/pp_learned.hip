@@ -50,7 +50,7 @@ __device__ __forceinline__ void term_features(const MlpDev& m, uint64_t x, uint6
         } else if (part == 2) {
             f[0] = (float)__popcll(x | z) / 16.f;
             f[1] = (float)(log(fmax(fabs(c), 1e-300)) / 8.0);
-            f[2] = remaining / m.depth_scale;
+            f[2] = fminf(remaining / m.depth_scale, 1.f);
         }
     }
 #pragma unroll
@@ -297,7 +297,11 @@ __global__ void __launch_bounds__(256) k_mlp_score(const MlpDev m, const ScoreAr
                       part, s.a0 + row * kMlpInPad);
         __syncthreads();
         mlp_forward_tile(m, s, nullptr);
-        if (threadIdx.x < kMlpTile && t0 + threadIdx.x < n) a.score[start + t0 + threadIdx.x] = s.y[threadIdx.x];
+        // score = ln|c| + the predicted ln|read-out factor| = the predicted ln|contribution|
+        if (threadIdx.x < kMlpTile && t0 + threadIdx.x < n) {
+            const uint64_t j = start + t0 + threadIdx.x;
+            a.score[j] = s.y[threadIdx.x] + (float)log(fabs((double)a.c[j]));
+        }
         __syncthreads();
     }
 }

@@ -66,26 +66,61 @@ __global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
         if (h[i]) atomicAdd(&a.hist[i], (unsigned long long)h[i]);
 }
 
-// One workgroup: pick the digit holding the k-th largest, then clear the histogram for the next pass.
+// One workgroup: pick the digit holding the k-th largest, then clear the histogram for the next pass.  The digit is
+// the largest d whose suffix count sum_{e >= d} hist[e] reaches k: each thread owns kPerThread consecutive bins, a
+// suffix scan of the per-thread sums (in LDS) finds the one thread whose bins hold d, and that thread walks them.
 __global__ void __launch_bounds__(kBlock) k_topn_select(TopnState* s, unsigned long long* hist, int pass, uint64_t n) {
-    if (threadIdx.x == 0 && !s->keep_all_finite) {
-        const int bins = 1 << topn_bits(pass);
-        if (pass == 0) {
-            unsigned long long total = 0;
-            for (int d = 0; d < bins; ++d) total += hist[d];
-            s->finite = total;
-            if (total <= n) s->keep_all_finite = 1;
+    constexpr int kPerThread = kTopnBins / kBlock;
+    static_assert(kPerThread * kBlock == kTopnBins, "bins split evenly over the workgroup");
+    __shared__ unsigned long long suffix[kBlock];
+    __shared__ int keep_all;
+    __shared__ unsigned long long k_sh;  // k as every thread must see it, read before the holder updates it
+    const int t = threadIdx.x;
+    const int bins = 1 << topn_bits(pass);
+    unsigned long long mine[kPerThread];
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        const int d = t * kPerThread + j;
+        mine[j] = d < bins ? hist[d] : 0ull;
+        sum += mine[j];
+    }
+    suffix[t] = sum;
+    __syncthreads();
+    // inclusive suffix scan: suffix[t] = sum over threads u >= t
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const unsigned long long v = t + off < kBlock ? suffix[t + off] : 0ull;
+        __syncthreads();
+        suffix[t] += v;
+        __syncthreads();
+    }
+    if (t == 0) {
+        int ka = s->keep_all_finite;
+        if (!ka && pass == 0) {
+            s->finite = suffix[0];
+            if (suffix[0] <= n) {
+                s->keep_all_finite = 1;
+                ka = 1;
+            }
         }
-        if (!s->keep_all_finite) {
-            unsigned long long acc = 0;
-            int chosen = 0;
-            for (int d = bins - 1; d >= 0; --d) {
-                const unsigned long long c = hist[d];
-                if (acc + c >= s->k) {
-                    chosen = d;
+        keep_all = ka;
+        k_sh = s->k;
+    }
+    __syncthreads();
+    if (!keep_all) {
+        const unsigned long long k = k_sh;
+        const unsigned long long above_mine = t + 1 < kBlock ? suffix[t + 1] : 0ull;  // bins of higher threads
+        // the thread holding the digit; with no digit reaching k (not reached in a selection) thread 0 takes bin 0
+        const bool holder = (above_mine < k && above_mine + sum >= k) || (t == 0 && suffix[0] < k);
+        if (holder) {
+            unsigned long long acc = above_mine;
+            int chosen = t * kPerThread;
+            for (int j = kPerThread - 1; j >= 0; --j) {
+                if (acc + mine[j] >= k) {
+                    chosen = t * kPerThread + j;
                     break;
                 }
-                acc += c;
+                acc += mine[j];
             }
             const int shift = topn_shift(pass);
             s->above += acc;
@@ -96,7 +131,7 @@ __global__ void __launch_bounds__(kBlock) k_topn_select(TopnState* s, unsigned l
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kTopnBins; i += kBlock) hist[i] = 0;
+    for (int i = t; i < kTopnBins; i += kBlock) hist[i] = 0;
 }
 
 template <int W, typename T>

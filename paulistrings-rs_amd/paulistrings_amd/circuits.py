@@ -99,6 +99,12 @@ class Circuit:
         self._rows.extend(other._rows)
         return self
 
+    def prefix(self, k: int) -> "Circuit":
+        """The first ``k`` channels as a circuit of their own."""
+        out = Circuit(self.num_qubits)
+        out._rows = self._rows[: max(0, int(k))]
+        return out
+
     def channels(self) -> np.ndarray:
         """The channels as a ``CHANNEL_DTYPE`` array, in circuit order."""
         out = np.zeros(len(self._rows), dtype=CHANNEL_DTYPE)

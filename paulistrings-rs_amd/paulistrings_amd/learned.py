@@ -4,9 +4,12 @@ and a layer pass that keeps the terms it ranks highest (pp_model_*, pp_truncate_
 No counterpart in the reference.  It plugs into the TruncationPolicy seam (crates/paulistrings/src/truncation/
 mod.rs:30-113); plain |c| TopN (truncation/builtin.rs:61-150) is both its fallback and its judge.
 
-Labels come from the plain engine: a term P sampled from a propagation with k layers still to apply is propagated on
-its own (coefficient 1) through those k layers, and its label is ln|c_P <psi|P_k|psi>| (floored), the logarithm of
-its share of the final read-out.  Many such single-term propagations run as one batch (pp_batch_propagate).
+The network predicts a term's read-out factor: for a term P with k layers still to apply, ln|<psi|P_k|psi>|, P_k
+being P propagated through those k layers.  The pass ranks terms by ln|c_P| + that prediction, the predicted ln of
+the term's share of the final read-out; a network that predicts a constant ranks exactly as |c| TopN does.
+
+Labels come from the plain engine: each sampled P is propagated on its own (coefficient 1) through its k layers and
+read out, floored at ``floor``.  Many such single-term propagations run as one batch (pp_batch_propagate).
 """
 
 from __future__ import annotations
@@ -160,25 +163,23 @@ def propagate_learned(sum_, circuit: Circuit, model: Model, keep: int, filter_po
                                                model._h, int(keep), ctypes.byref(opts), None), sum_.ctx.handle)
 
 
-def contribution_labels(terms_x, terms_z, terms_c, circuit_rest: Circuit, state: str, policy=None,
-                        direction: str = "heisenberg", batch: int = 256) -> np.ndarray:
-    """ln|c_P <psi| P propagated through circuit_rest |psi>| for each term P (floored at LABEL_FLOOR), each term
-    propagated on its own by the plain engine, ``batch`` terms per pp_batch_propagate call."""
+def readout_labels(terms_x, terms_z, circuit_rest: Circuit, state: str, policy=None, direction: str = "heisenberg",
+                   floor: float = LABEL_FLOOR, batch: int = 256) -> np.ndarray:
+    """ln|<psi| P propagated through circuit_rest |psi>| for each one-word term P (floored at ``floor``), each term
+    propagated on its own with coefficient 1 by the plain engine, ``batch`` terms per pp_batch_propagate call."""
     from . import PauliSum, propagate_batch
 
     x = np.asarray(terms_x, np.uint64).reshape(-1)
     z = np.asarray(terms_z, np.uint64).reshape(-1)
-    c = np.asarray(terms_c, np.float64).reshape(-1)
     nq = circuit_rest.num_qubits
-    out = np.full(len(x), LABEL_FLOOR, np.float32)
+    out = np.full(len(x), floor, np.float32)
     for lo in range(0, len(x), batch):
         hi = min(len(x), lo + batch)
         sums = [PauliSum(nq, x[i:i + 1].reshape(1, 1), z[i:i + 1].reshape(1, 1), np.array([1.0])).to_device(0)
                 for i in range(lo, hi)]
         propagate_batch(sums, [circuit_rest] * len(sums), [policy] * len(sums), [direction] * len(sums))
         for k, g in enumerate(sums):
-            f = g.expectation(state)
-            v = abs(c[lo + k] * f)
-            out[lo + k] = max(LABEL_FLOOR, math.log(v)) if v > 0 else LABEL_FLOOR
+            v = abs(g.expectation(state))
+            out[lo + k] = max(floor, math.log(v)) if v > 0 else floor
             g.free()
     return out

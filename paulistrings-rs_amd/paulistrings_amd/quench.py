@@ -1,0 +1,118 @@
+"""Learned truncation against |c| TopN on the periodic 8 x 8 transverse-field Ising quench (64 qubits, one word).
+
+    python -m paulistrings_amd.quench --h 1.0 --steps 40 --keep 65536 --judge 1048576
+
+The average-X observable is propagated (Heisenberg) through ``steps`` Trotter steps (dt = 0.05, J = 1) three ways:
+|c| TopN(judge) as the reference value, |c| TopN(keep), and the learned pass keeping ``keep`` terms.  The network is
+trained first, on labels from plain short runs of the same lattice: terms sampled from a |c| TopN propagation of
+``train_steps`` steps, each labelled with ln|its read-out factor in that short run| (the plain engine propagating it
+alone, coefficient 1, through the layers left, threshold ``label_eps``; floored at ln(label_eps)).  Prints one JSON line: the three read-outs, the two errors against
+the judge, and each run's wall time.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import PauliSum, truncation
+from .circuits import repeat, tfim_square_step
+from .learned import Model, lattice_groups, propagate_learned, readout_labels
+
+
+def average_x(n: int) -> PauliSum:
+    x = (np.uint64(1) << np.arange(n, dtype=np.uint64)).reshape(-1, 1)
+    return PauliSum(n, x, np.zeros((n, 1), np.uint64), np.full(n, 1.0 / n))
+
+
+def training_set(step, train_steps: int, keep: int, per_checkpoint: int, eps: float, label_eps: float, seed: int):
+    """(x, z, c, remaining, label) sampled from a |c| TopN(keep) run of train_steps steps, at every step boundary."""
+    rng = np.random.default_rng(seed)
+    n = step.num_qubits
+    short = repeat(step, train_steps)
+    per = len(step)
+    xs, zs, cs, rs, ls = [], [], [], [], []  # noqa: E702
+    g = average_x(n).to_device(0)
+    done = 0
+    for k in range(train_steps):
+        # Heisenberg order: the last step first; after k + 1 steps, the first (train_steps - k - 1) steps remain
+        g.propagate(step, truncation.coeff(eps) & truncation.topn(keep), "heisenberg")
+        done += per
+        rest = short.prefix(len(short) - done)
+        h = g.to_host()
+        x, z, c = h.x_array()[:, 0], h.z_array()[:, 0], h.coefficients_array()
+        if len(c) == 0 or len(rest) == 0:
+            continue
+        take = min(per_checkpoint, len(c))
+        top = np.argsort(-np.abs(c))[: take // 2]
+        rnd = rng.choice(len(c), size=take - len(top), replace=len(c) < take - len(top))
+        idx = np.concatenate([top, rnd])
+        labels = readout_labels(x[idx], z[idx], rest, "x+", truncation.coeff(label_eps), floor=math.log(label_eps))
+        xs.append(x[idx]); zs.append(z[idx]); cs.append(c[idx])
+        rs.append(np.full(len(idx), len(rest), np.float32)); ls.append(labels)
+    g.free()
+    return (np.concatenate(xs), np.concatenate(zs), np.concatenate(cs), np.concatenate(rs), np.concatenate(ls))
+
+
+def run(h: float, steps: int, keep: int, judge: int, train_steps: int, per_checkpoint: int, epochs: int,
+        eps: float, label_eps: float, seed: int) -> dict:
+    n = 64
+    step = tfim_square_step(8, 8, dt=0.05, j=1.0, h=h)
+    circ = repeat(step, steps)
+    t0 = time.perf_counter()
+    x, z, c, rem, lab = training_set(step, train_steps, keep, per_checkpoint, eps, label_eps, seed)
+    t_labels = time.perf_counter() - t0
+    # the horizon: the longest remaining depth the labels cover; deeper terms read as that depth
+    model = Model(lattice_groups(8, 8), depth_scale=float(max(1, (train_steps - 1) * len(step))), seed=seed)
+    t0 = time.perf_counter()
+    loss = model.train(x, z, c, rem, lab, epochs=epochs, batch=512, lr=1e-3, seed=seed)
+    t_train = time.perf_counter() - t0
+
+    def timed(fn):
+        g = average_x(n).to_device(0)
+        g.ctx.sync()
+        t = time.perf_counter()
+        fn(g)
+        v = g.expectation("x+")
+        dt = time.perf_counter() - t
+        g.free()
+        return v, dt
+
+    judge_v, judge_t = timed(lambda g: g.propagate(circ, truncation.coeff(eps) & truncation.topn(judge), "heisenberg"))
+    plain_v, plain_t = timed(lambda g: g.propagate(circ, truncation.coeff(eps) & truncation.topn(keep), "heisenberg"))
+    learn_v, learn_t = timed(lambda g: propagate_learned(g, circ, model, keep, truncation.coeff(eps), "heisenberg"))
+    model.free()
+    return {
+        "lattice": "8x8 periodic TFIM", "h": h, "dt": 0.05, "steps": steps, "layers": len(circ), "keep": keep,
+        "judge_keep": judge, "threshold": eps, "judge": judge_v, "plain_topn": plain_v, "learned": learn_v,
+        "err_plain": abs(plain_v - judge_v), "err_learned": abs(learn_v - judge_v), "seconds_judge": judge_t,
+        "seconds_plain": plain_t, "seconds_learned": learn_t, "train_samples": int(len(lab)),
+        "train_loss": loss, "seconds_labels": t_labels, "seconds_training": t_train,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--h", type=float, default=1.0)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--keep", type=int, default=1 << 16)
+    ap.add_argument("--judge", type=int, default=1 << 20)
+    ap.add_argument("--train-steps", type=int, default=4)
+    ap.add_argument("--per-checkpoint", type=int, default=2048)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--eps", type=float, default=1e-7)
+    ap.add_argument("--label-eps", type=float, default=1e-5, help="threshold of the single-term label runs")
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args()
+    print(json.dumps(run(a.h, a.steps, a.keep, a.judge, a.train_steps, a.per_checkpoint, a.epochs, a.eps, a.label_eps, a.seed)),
+          flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -99,7 +99,8 @@ def test_learned_pass_keeps_the_top_scores(n):
     g = s.to_device(0)
     h = g.to_host()  # the canonical order
     hx, hz, hc = h.x_array()[:, 0], h.z_array()[:, 0], h.coefficients_array()
-    score = m.predict(hx, hz, hc, np.float32(12))
+    # the pass ranks by ln|c| + the network's output, summed in fp32
+    score = m.predict(hx, hz, hc, np.float32(12)) + np.log(np.abs(hc)).astype(np.float32)
     key = score.astype(np.float64)
     order = sorted(range(len(key)), key=lambda i: -key[i])  # stable: ties keep the canonical order
     keep = sorted(order[:n])

@@ -32,6 +32,13 @@ def test_features_count_rows_and_columns():
     assert (f[35:] == 0).all()
 
 
+def test_remaining_feature_saturates_at_the_horizon():
+    groups = L.lattice_groups(4, 4)
+    rem = np.array([0.0, 20.0, 40.0, 41.0, 1000.0], np.float32)
+    f = L.features(np.zeros(5, np.uint64), np.zeros(5, np.uint64), np.ones(5), groups, rem, 40.0)
+    assert list(f[:, 34]) == [0.0, 0.5, 1.0, 1.0, 1.0]
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_backward_matches_finite_differences(seed):
     rng = np.random.default_rng(seed)

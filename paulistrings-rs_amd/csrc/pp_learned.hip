@@ -1,11 +1,14 @@
 // pp_learned.hip -- the learned truncation policy's network: features, a 35-128-128-1 MLP whose two hidden GEMMs
 // run on bf16 MFMA (v_mfma_f32_16x16x32_bf16) with fp32 accumulation, its backward pass, Adam, and the per-term score.
 //
-// A workgroup of four waves takes a tile of 64 terms; wave w owns rows 16w .. 16w + 15 of every activation matrix,
-// so a layer's output tile feeds the next layer within the wave.  MFMA operand layout (16x16x32, bf16): lane l
+// A workgroup of four waves takes a tile of 64 terms.  In the forward pass wave w owns hidden columns 32w .. 32w + 31
+// of both layers, with those columns' weight fragments in registers for the whole kernel, and reads the activation
+// rows from LDS; the backward's GEMMs read both operands from LDS or global memory.  MFMA operand layout (16x16x32, bf16): lane l
 // supplies row (l & 15) of A and column (l & 15) of B, each at k = 8 (l >> 4) .. 8 (l >> 4) + 7; it receives
 // D[4 (l >> 4) + q][l & 15] for q = 0..3.  Both operands are read with k contiguous: A row-major, B as its transpose.
 #include "pp_learned.h"
+
+#include <algorithm>
 
 namespace pp {
 
@@ -34,80 +37,193 @@ __device__ __forceinline__ f32x4 mfma_16x16x32(const uint16_t* a, int lda, const
                                                     acc, 0, 0, 0);
 }
 
-// The 64 bf16 features of one term into row[0 .. 64): four threads per term, `part` = which 16.
-__device__ __forceinline__ void term_features(const MlpDev& m, uint64_t x, uint64_t z, double c, float remaining,
-                                              bool valid, int part, uint16_t* row) {
-    float f[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) f[i] = 0.f;
-    if (valid) {
-        if (part == 0) {
-#pragma unroll
-            for (int g = 0; g < 16; ++g) f[g] = (float)__popcll(x & m.groups[g]) / 8.f;
-        } else if (part == 1) {
-#pragma unroll
-            for (int g = 0; g < 16; ++g) f[g] = (float)__popcll(z & m.groups[g]) / 8.f;
-        } else if (part == 2) {
-            f[0] = (float)__popcll(x | z) / 16.f;
-            f[1] = (float)(log(fmax(fabs(c), 1e-300)) / 8.0);
-            f[2] = fminf(remaining / m.depth_scale, 1.f);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 16; ++i) row[16 * part + i] = bf16_bits(f[i]);
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// Set bits of a 64-bit word as a 32-bit count (two v_bcnt; the 64-bit popcount's result would be converted to
+// float as a 64-bit integer).
+__device__ __forceinline__ uint32_t popc64(uint64_t v) { return __popc((uint32_t)v) + __popc((uint32_t)(v >> 32)); }
+
+// max(v, 0) as one integer max on the bits: a float with the sign bit set is a negative int.  Equal to fmaxf(v, 0)
+// except that -0 and negative NaNs give +0 (a NaN with the sign clear stays NaN).
+__device__ __forceinline__ float relu(float v) { return __int_as_float(max(__float_as_int(v), 0)); }
+
+// Four floats to bf16 (round to nearest even, v_cvt_pk_bf16_f32), packed in 8 bytes.
+__device__ __forceinline__ uint2 pack_bf16x4(float a, float b, float c, float d) {
+    bf16x4 v;
+    v[0] = (__bf16)a;
+    v[1] = (__bf16)b;
+    v[2] = (__bf16)c;
+    v[3] = (__bf16)d;
+    return __builtin_bit_cast(uint2, v);
 }
 
+// The 64 bf16 features of one term into row[0 .. 64), one wave per part: the term is lane (threadIdx & 63) and
+// part = threadIdx >> 6 picks which 16 features the thread writes -- 0: X occupancies, 1: Z occupancies, 2: the
+// scalar features, 3: padding -- so every branch below is uniform across a wave.  Part 2 also leaves
+// ln max(|c|, 1e-300) in *lnc when lnc is not null.
+__device__ __forceinline__ void term_features(const MlpDev& m, uint64_t x, uint64_t z, double c, float remaining,
+                                              bool valid, int part, uint16_t* row, float* lnc) {
+    float f[16];
+    if (part < 2) {
+        const uint64_t v = part == 0 ? x : z;
+#pragma unroll
+        for (int g = 0; g < 16; ++g) f[g] = valid ? (float)popc64(v & m.groups[g]) * 0.125f : 0.f;
+    } else {
+#pragma unroll
+        for (int g = 0; g < 16; ++g) f[g] = 0.f;
+        if (part == 2) {
+            const float L = (float)log(fmax(fabs(c), 1e-300));
+            if (lnc) *lnc = L;
+            if (valid) {
+                f[0] = (float)popc64(x | z) / 16.f;
+                f[1] = L / 8.f;  // exact: a power-of-two scale of a value far from the subnormal range
+                f[2] = fminf(remaining / m.depth_scale, 1.f);
+            }
+        }
+    }
+    uint4 lo, hi;
+    const uint2 p0 = pack_bf16x4(f[0], f[1], f[2], f[3]), p1 = pack_bf16x4(f[4], f[5], f[6], f[7]);
+    const uint2 p2 = pack_bf16x4(f[8], f[9], f[10], f[11]), p3 = pack_bf16x4(f[12], f[13], f[14], f[15]);
+    lo = make_uint4(p0.x, p0.y, p1.x, p1.y);
+    hi = make_uint4(p2.x, p2.y, p3.x, p3.y);
+    reinterpret_cast<uint4*>(row + 16 * part)[0] = lo;
+    reinterpret_cast<uint4*>(row + 16 * part)[1] = hi;
+}
+
+// Row strides (bf16 elements) of the activation tiles in LDS, padded by 16 bytes so that the 16 rows one MFMA operand
+// read touches start in different banks.
+constexpr int kLd0 = kMlpInPad + 8, kLd1 = kMlpHid + 8;
+
 // Shared memory of a forward tile.
-struct FwdLds {
-    uint16_t a0[kMlpTile * kMlpInPad];  // features, bf16
-    uint16_t a1[kMlpTile * kMlpHid];    // bf16 relu(z1)
+struct alignas(16) FwdLds {
+    uint16_t a0[kMlpTile * kLd0];  // features, bf16 [row][input]
+    uint16_t a1[kMlpTile * kLd1];  // bf16 relu(z1) [row][hidden]
+    float ypart[2][kWaves][kMlpTile];  // each wave's share of the output dot product, by tile parity
     float y[kMlpTile];
 };
 
-// Layers 1 and 2 and the output for the tile in s.a0; h2 (fp32 relu(z2)) is written when h2 is not null.
-__device__ __forceinline__ void mlp_forward_tile(const MlpDev& m, FwdLds& s, float* h2) {
+// The weights a wave multiplies by, held in registers for the whole kernel.  The GEMMs are computed transposed,
+// Z^T = W^T A^T, with the weights as the MFMA's A operand and the terms as its B operand: wave w owns hidden units
+// 32w .. 32w + 31 (two 16-row tiles j), lane l supplies weight row 32w + 16j + (l & 15) and receives, for term
+// (l & 15) of the term tile, the four consecutive hidden units 32w + 16j + 4 (l >> 4) + q, which are contiguous in
+// the next layer's [term][hidden] operand.
+struct FwdWeights {
+    bf16x8 w1[2][kMlpInPad / 32];  // A operand fragments of W1^T, per hidden tile and k step
+    bf16x8 w2[2][kMlpHid / 32];
+    float b1[2][4], b2[2][4], w3[2][4];  // at the hidden units this lane receives
+    float b3;
+};
+
+__device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+
+__device__ __forceinline__ FwdWeights load_fwd_weights(const MlpDev& m) {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const float* b1 = m.params + kOffB1;
-    const float* b2 = m.params + kOffB2;
-    const float* w3 = m.params + kOffW3;
-    // layer 1: rows 16w .. 16w + 15, all 128 outputs, K = 64
-    for (int ct = 0; ct < kMlpHid / 16; ++ct) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int lr = l & 15, lk = (l >> 4) * 8, lq = 4 * (l >> 4);
+    FwdWeights f;
 #pragma unroll
-        for (int ks = 0; ks < kMlpInPad / 32; ++ks)
-            acc = mfma_16x16x32(s.a0 + (16 * w) * kMlpInPad + 32 * ks, kMlpInPad, m.w1t + (16 * ct) * kMlpInPad + 32 * ks,
-                                kMlpInPad, acc);
-        const int col = 16 * ct + (l & 15);
+    for (int j = 0; j < 2; ++j) {
+        const int row = 32 * w + 16 * j + lr;
+#pragma unroll
+        for (int ks = 0; ks < kMlpInPad / 32; ++ks) f.w1[j][ks] = load_frag(m.w1t + row * kMlpInPad + 32 * ks + lk);
+#pragma unroll
+        for (int ks = 0; ks < kMlpHid / 32; ++ks) f.w2[j][ks] = load_frag(m.w2t + row * kMlpHid + 32 * ks + lk);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int row = 16 * w + 4 * (l >> 4) + q;
-            s.a1[row * kMlpHid + col] = bf16_bits(fmaxf(acc[q] + b1[col], 0.f));
+            const int u = 32 * w + 16 * j + lq + q;
+            f.b1[j][q] = m.params[kOffB1 + u];
+            f.b2[j][q] = m.params[kOffB2 + u];
+            f.w3[j][q] = m.params[kOffW3 + u];
         }
     }
+    f.b3 = m.params[kOffB3];
+    return f;
+}
+
+// Layer 1 of the tile in s.a0 into s.a1 (K = 64).  The eight (term tile, hidden tile) accumulators are independent
+// MFMA chains, stepped through K together.
+__device__ __forceinline__ void mlp_layer1(const FwdWeights& fw, FwdLds& s) {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lr = l & 15, lk = (l >> 4) * 8, lq = 4 * (l >> 4);
+    f32x4 acc[kMlpTile / 16][2];
+#pragma unroll
+    for (int i = 0; i < kMlpTile / 16; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{fw.b1[j][0], fw.b1[j][1], fw.b1[j][2], fw.b1[j][3]};
+#pragma unroll
+    for (int ks = 0; ks < kMlpInPad / 32; ++ks) {
+#pragma unroll
+        for (int i = 0; i < kMlpTile / 16; ++i) {
+            const bf16x8 a = load_frag(s.a0 + (16 * i + lr) * kLd0 + 32 * ks + lk);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw.w1[j][ks], a, acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kMlpTile / 16; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            *reinterpret_cast<uint2*>(s.a1 + (16 * i + lr) * kLd1 + 32 * w + 16 * j + lq) =
+                pack_bf16x4(relu(acc[i][j][0]), relu(acc[i][j][1]), relu(acc[i][j][2]), relu(acc[i][j][3]));
+}
+
+// Layer 2 of the tile in s.a1 (K = 128) and this wave's share of the output into ypart[w]; h2 (fp32 relu(z2),
+// [row][hidden]) is written when not null.
+__device__ __forceinline__ void mlp_layer2(const FwdWeights& fw, const FwdLds& s, float* h2,
+                                           float (*ypart)[kMlpTile]) {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lr = l & 15, lk = (l >> 4) * 8, lq = 4 * (l >> 4);
+    f32x4 acc[kMlpTile / 16][2];
+#pragma unroll
+    for (int i = 0; i < kMlpTile / 16; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{fw.b2[j][0], fw.b2[j][1], fw.b2[j][2], fw.b2[j][3]};
+#pragma unroll
+    for (int ks = 0; ks < kMlpHid / 32; ++ks) {
+#pragma unroll
+        for (int i = 0; i < kMlpTile / 16; ++i) {
+            const bf16x8 a = load_frag(s.a1 + (16 * i + lr) * kLd1 + 32 * ks + lk);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw.w2[j][ks], a, acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kMlpTile / 16; ++i) {
+        float part = 0.f;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            float h[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                h[q] = relu(acc[i][j][q]);
+                part = __builtin_fmaf(h[q], fw.w3[j][q], part);
+            }
+            if (h2)
+                *reinterpret_cast<float4*>(h2 + (16 * i + lr) * kMlpHid + 32 * w + 16 * j + lq) =
+                    make_float4(h[0], h[1], h[2], h[3]);
+        }
+        part += __shfl_xor(part, 16, 64);
+        part += __shfl_xor(part, 32, 64);
+        ypart[w][16 * i + lr] = part;  // the four lanes of a term hold the same sum
+    }
+}
+
+// The output of term t: the waves' shares in order, then b3.
+__device__ __forceinline__ float mlp_output(const FwdWeights& fw, const float (*ypart)[kMlpTile], int t) {
+    float v = ypart[0][t];
+#pragma unroll
+    for (int ww = 1; ww < kWaves; ++ww) v += ypart[ww][t];
+    return v + fw.b3;
+}
+
+// Layers 1 and 2 and the output for the tile in s.a0, into s.y; h2 as in mlp_layer2.
+__device__ __forceinline__ void mlp_forward_tile(const FwdWeights& fw, FwdLds& s, float* h2) {
+    mlp_layer1(fw, s);
     __syncthreads();
-    // layer 2 and the output: K = 128
-    float part[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int ct = 0; ct < kMlpHid / 16; ++ct) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < kMlpHid / 32; ++ks)
-            acc = mfma_16x16x32(s.a1 + (16 * w) * kMlpHid + 32 * ks, kMlpHid, m.w2t + (16 * ct) * kMlpHid + 32 * ks,
-                                kMlpHid, acc);
-        const int col = 16 * ct + (l & 15);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float h = fmaxf(acc[q] + b2[col], 0.f);
-            if (h2) h2[(16 * w + 4 * (l >> 4) + q) * kMlpHid + col] = h;
-            part[q] += h * w3[col];
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        float v = part[q];
-#pragma unroll
-        for (int d = 1; d < 16; d <<= 1) v += __shfl_xor(v, d, 64);
-        if ((l & 15) == 0) s.y[16 * w + 4 * (l >> 4) + q] = v + m.params[kOffB3];
-    }
+    mlp_layer2(fw, s, h2, s.ypart[0]);
+    __syncthreads();
+    if (threadIdx.x < kMlpTile) s.y[threadIdx.x] = mlp_output(fw, s.ypart[0], threadIdx.x);
     __syncthreads();
 }
 
@@ -128,14 +244,15 @@ __global__ void __launch_bounds__(256) k_mlp_refresh(const float* __restrict__ p
 
 __global__ void __launch_bounds__(256) k_mlp_predict(const MlpDev m, const MlpBatch b, float* __restrict__ out) {
     __shared__ FwdLds s;
+    const FwdWeights fw = load_fwd_weights(m);
     const uint64_t base = (uint64_t)blockIdx.x * kMlpTile;
-    const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
+    const int row = threadIdx.x & 63, part = threadIdx.x >> 6;
     const uint64_t i = base + row;
     const bool valid = i < b.count;
     term_features(m, valid ? b.x[i] : 0, valid ? b.z[i] : 0, valid ? b.c[i] : 1.0, valid ? b.remaining[i] : 0.f,
-                  valid, part, s.a0 + row * kMlpInPad);
+                  valid, part, s.a0 + row * kLd0, nullptr);
     __syncthreads();
-    mlp_forward_tile(m, s, nullptr);
+    mlp_forward_tile(fw, s, nullptr);
     if (threadIdx.x < kMlpTile && base + threadIdx.x < b.count) out[base + threadIdx.x] = s.y[threadIdx.x];
 }
 
@@ -161,17 +278,18 @@ __device__ __forceinline__ void add_column_sums(const float* h, float* g) {
 __global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch b, float* __restrict__ grads,
                                                   double* __restrict__ loss_sum) {
     __shared__ GradLds s;
+    const FwdWeights fw = load_fwd_weights(m);
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t base = (uint64_t)blockIdx.x * kMlpTile;
     {
-        const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
+        const int row = threadIdx.x & 63, part = threadIdx.x >> 6;
         const uint64_t i = base + row;
         const bool valid = i < b.count;
         term_features(m, valid ? b.x[i] : 0, valid ? b.z[i] : 0, valid ? b.c[i] : 1.0, valid ? b.remaining[i] : 0.f,
-                      valid, part, s.f.a0 + row * kMlpInPad);
+                      valid, part, s.f.a0 + row * kLd0, nullptr);
     }
     __syncthreads();
-    mlp_forward_tile(m, s.f, s.h);
+    mlp_forward_tile(fw, s.f, s.h);
     // dy and the loss
     if (threadIdx.x < kMlpTile) {
         const uint64_t i = base + threadIdx.x;
@@ -186,11 +304,11 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch
     // transposes of a0 and a1 for the weight-gradient GEMMs
     for (int i = threadIdx.x; i < kMlpTile * kMlpInPad; i += 256) {
         const int r = i / kMlpInPad, k = i % kMlpInPad;
-        s.a0t[k * kMlpTile + r] = s.f.a0[i];
+        s.a0t[k * kMlpTile + r] = s.f.a0[r * kLd0 + k];
     }
     for (int i = threadIdx.x; i < kMlpTile * kMlpHid; i += 256) {
         const int r = i / kMlpHid, k = i % kMlpHid;
-        s.a1t[k * kMlpTile + r] = s.f.a1[i];
+        s.a1t[k * kMlpTile + r] = s.f.a1[r * kLd1 + k];
     }
     __syncthreads();
     // w3, b3 gradients; dz2 = dy w3 (h2 > 0)
@@ -242,7 +360,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int r = 16 * w + 4 * (l >> 4) + q;
-            const float d = bf16_float(s.f.a1[r * kMlpHid + i]) > 0.f ? acc[q] : 0.f;
+            const float d = bf16_float(s.f.a1[r * kLd1 + i]) > 0.f ? acc[q] : 0.f;
             s.h[r * kMlpHid + i] = d;
             s.dzt[i * kMlpTile + r] = bf16_bits(d);
         }
@@ -282,28 +400,87 @@ __global__ void __launch_bounds__(256) k_mlp_adam(float* __restrict__ p, float* 
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// The first nonempty bucket at or after b among this workgroup's (b, b + gridDim.x, ...); num_buckets when none.
 template <typename T>
-__global__ void __launch_bounds__(256) k_mlp_score(const MlpDev m, const ScoreArgs<T> a) {
-    __shared__ FwdLds s;
-    const uint32_t b = blockIdx.x;
-    const uint32_t n = a.len[b];
-    const uint64_t start = a.start[b];
-    const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
-    for (uint32_t t0 = 0; t0 < n; t0 += kMlpTile) {
-        const uint32_t r = t0 + row;
-        const bool valid = r < n;
-        const uint64_t i = start + r;
-        term_features(m, valid ? a.x[i] : 0, valid ? a.z[i] : 0, valid ? (double)a.c[i] : 1.0, a.remaining, valid,
-                      part, s.a0 + row * kMlpInPad);
-        __syncthreads();
-        mlp_forward_tile(m, s, nullptr);
-        // score = ln|c| + the predicted ln|read-out factor| = the predicted ln|contribution|
-        if (threadIdx.x < kMlpTile && t0 + threadIdx.x < n) {
-            const uint64_t j = start + t0 + threadIdx.x;
-            a.score[j] = s.y[threadIdx.x] + (float)log(fabs((double)a.c[j]));
+__device__ __forceinline__ uint32_t next_bucket(const ScoreArgs<T>& a, uint32_t b, uint32_t& n, uint64_t& start) {
+    for (; b < a.num_buckets; b += gridDim.x) {
+        n = a.len[b];
+        if (n) {
+            start = a.start[b];
+            return b;
         }
-        __syncthreads();
     }
+    return b;
+}
+
+// The words a thread's part of the features reads (x for parts 0 and 2, z for 1 and 2, c for 2).
+template <typename T>
+__device__ __forceinline__ void load_term(const ScoreArgs<T>& a, uint64_t i, bool valid, int part, uint64_t& x,
+                                          uint64_t& z, double& c) {
+    x = 0;
+    z = 0;
+    c = 1.0;
+    if (!valid) return;
+    if (part == 0 || part == 2) x = a.x[i];
+    if (part == 1 || part == 2) z = a.z[i];
+    if (part == 2) c = (double)a.c[i];
+}
+
+// Workgroups stride over the buckets, each loading its weight fragments once and walking its buckets' 64-term tiles
+// with the next tile's words loaded while the current tile is scored.  Two barriers per tile: a tile's output sum
+// and score are written after the next tile's first barrier, from the parity-indexed ypart and lnc.
+// Four waves per SIMD (128 VGPRs, no spills): all of a grid of 4 workgroups per CU is resident.
+template <typename T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_mlp_score(const MlpDev m, const ScoreArgs<T> a) {
+    __shared__ FwdLds s;
+    __shared__ float lnc[2][kMlpTile];  // ln max(|c|, 1e-300) by tile parity
+    const FwdWeights fw = load_fwd_weights(m);
+    const int r = threadIdx.x & 63, part = threadIdx.x >> 6;
+    uint32_t n = 0;
+    uint64_t start = 0;
+    uint32_t b = next_bucket(a, blockIdx.x, n, start);
+    if (b >= a.num_buckets) return;  // uniform: depends on blockIdx only
+    uint32_t t0 = 0;
+    uint64_t cx, cz;
+    double cc;
+    load_term(a, start + r, r < n, part, cx, cz, cc);
+    uint64_t prev = 0;  // column position of the previous tile's term r, when prev_valid
+    bool prev_valid = false;
+    int parity = 0;
+    for (;; parity ^= 1) {
+        uint32_t nb = b, nt = t0 + kMlpTile, nn = n;
+        uint64_t ns = start;
+        if (nt >= n) {
+            nt = 0;
+            nb = next_bucket(a, b + gridDim.x, nn, ns);
+        }
+        const bool more = nb < a.num_buckets;
+        uint64_t px = 0, pz = 0;
+        double pc = 1.0;
+        if (more) load_term(a, ns + nt + r, nt + r < nn, part, px, pz, pc);
+        const bool valid = t0 + r < n;
+        term_features(m, cx, cz, cc, a.remaining, valid, part, s.a0 + r * kLd0, part == 2 ? &lnc[parity][r] : nullptr);
+        __syncthreads();
+        // score = ln|c| + the predicted ln|read-out factor| = the predicted ln|contribution|
+        if (part == 0 && prev_valid)
+            a.score[prev] = mlp_output(fw, s.ypart[parity ^ 1], r) + lnc[parity ^ 1][r];
+        mlp_layer1(fw, s);
+        __syncthreads();
+        mlp_layer2(fw, s, nullptr, s.ypart[parity]);
+        prev = start + t0 + r;
+        prev_valid = valid;
+        if (!more) break;
+        b = nb;
+        t0 = nt;
+        n = nn;
+        start = ns;
+        cx = px;
+        cz = pz;
+        cc = pc;
+    }
+    __syncthreads();
+    if (part == 0 && prev_valid) a.score[prev] = mlp_output(fw, s.ypart[parity], r) + lnc[parity][r];
 }
 
 }  // namespace
@@ -338,7 +515,16 @@ hipError_t launch_mlp_adam(hipStream_t st, float* params, float* adam_m, float* 
 template <typename T>
 hipError_t launch_mlp_score(hipStream_t st, const MlpDev& m, const ScoreArgs<T>& a) {
     if (a.num_buckets == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_mlp_score<T>), dim3(a.num_buckets), dim3(256), 0, st, m, a);
+    static int cus_of[64] = {};  // compute units per device, read once
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    if (!cus_of[dev]) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        cus_of[dev] = cus;
+    }
+    const uint32_t grid = std::min<uint32_t>(a.num_buckets, (uint32_t)cus_of[dev] * 4);
+    hipLaunchKernelGGL((k_mlp_score<T>), dim3(grid), dim3(256), 0, st, m, a);
     return hipGetLastError();
 }
 

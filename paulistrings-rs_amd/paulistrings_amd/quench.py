@@ -25,6 +25,10 @@ from .circuits import repeat, tfim_square_step
 from .learned import Model, lattice_groups, propagate_learned, readout_labels
 
 
+def _progress(msg: str) -> None:
+    print(f"[quench] {msg}", file=sys.stderr, flush=True)
+
+
 def average_x(n: int) -> PauliSum:
     x = (np.uint64(1) << np.arange(n, dtype=np.uint64)).reshape(-1, 1)
     return PauliSum(n, x, np.zeros((n, 1), np.uint64), np.full(n, 1.0 / n))
@@ -67,11 +71,13 @@ def run(h: float, steps: int, keep: int, judge: int, train_steps: int, per_check
     t0 = time.perf_counter()
     x, z, c, rem, lab = training_set(step, train_steps, keep, per_checkpoint, eps, label_eps, seed)
     t_labels = time.perf_counter() - t0
+    _progress(f"{len(lab)} labels in {t_labels:.1f} s")
     # the horizon: the longest remaining depth the labels cover; deeper terms read as that depth
     model = Model(lattice_groups(8, 8), depth_scale=float(max(1, (train_steps - 1) * len(step))), seed=seed)
     t0 = time.perf_counter()
     loss = model.train(x, z, c, rem, lab, epochs=epochs, batch=512, lr=1e-3, seed=seed)
     t_train = time.perf_counter() - t0
+    _progress(f"trained to loss {loss:.3f} in {t_train:.1f} s")
 
     def timed(fn):
         g = average_x(n).to_device(0)
@@ -81,6 +87,7 @@ def run(h: float, steps: int, keep: int, judge: int, train_steps: int, per_check
         v = g.expectation("x+")
         dt = time.perf_counter() - t
         g.free()
+        _progress(f"run done: {v:.12f} in {dt:.2f} s")
         return v, dt
 
     judge_v, judge_t = timed(lambda g: g.propagate(circ, truncation.coeff(eps) & truncation.topn(judge), "heisenberg"))

@@ -179,6 +179,158 @@ def cpu_baseline(channels, dtype_name: str, log2n: int, budget_s: float):
     }
 
 
+MLP_FLOP_PER_TERM = 2 * (35 * 128 + 128 * 128 + 128)  # algorithmic: 35 -> 128 -> 128 -> 1
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def learned_score_line(steps: int, warmup: int, log2n: int) -> dict:
+    """The learned pass on 2^log2n resident one-word terms (64 qubits, bits Bernoulli(1/4), N(0, 1) coefficients):
+    every step scores all terms with the network (k_mlp_score) and keeps all but one by score.  The roofline is the
+    score kernel's, from hipEvents on the library's stream; value is terms scored per second of whole passes."""
+    import numpy as np
+
+    from paulistrings_amd import PauliSum
+    from paulistrings_amd.learned import Model, lattice_groups, truncate_learned
+
+    rng = np.random.default_rng(SEED)
+    n = 1 << log2n
+    x = rng.integers(0, 2**63, size=n, dtype=np.uint64) & rng.integers(0, 2**63, size=n, dtype=np.uint64)
+    z = rng.integers(0, 2**63, size=n, dtype=np.uint64) & rng.integers(0, 2**63, size=n, dtype=np.uint64)
+    c = rng.standard_normal(n)
+    g = PauliSum(64, x.reshape(-1, 1), z.reshape(-1, 1), c).to_device(0)
+    model = Model(lattice_groups(8, 8), depth_scale=576.0, seed=1)
+    ctx = g.ctx
+    for _ in range(warmup):
+        truncate_learned(g, model, len(g) - 1, 100.0)
+    ctx.profile_enable(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        truncate_learned(g, model, len(g) - 1, 100.0)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    launches, ms, terms = ctx.profile_read_score()
+    ctx.profile_enable(False)
+    model.free()
+    g.free()
+    tflops = terms * MLP_FLOP_PER_TERM / (ms / 1e3) / 1e12 if ms > 0 else 0.0
+    cpu = learned_cpu_baseline()
+    return {
+        "metric": "learned-pass scored terms/s",
+        "value": terms / wall,
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"learned TopN pass over 2^{log2n} one-word terms (64 qubits), 35-128-128-1 MLP",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None},
+        "cpu_baseline": cpu,
+        "detail": {"kernel": "k_mlp_score", "launches": launches, "kernel_avg_ms": ms / max(1, launches),
+                   "terms_per_launch": terms / max(1, launches), "flop_per_term": MLP_FLOP_PER_TERM},
+    }
+
+
+def learned_cpu_baseline() -> dict:
+    """The oracle's numpy forward (the same bf16-rounded network) on a 2^16-term sample."""
+    import numpy as np
+    from threadpoolctl import threadpool_info
+
+    import learned as lo
+
+    rng = np.random.default_rng(SEED ^ 0x5A5A)
+    n = 1 << 16
+    x = rng.integers(0, 2**63, size=n, dtype=np.uint64) & rng.integers(0, 2**63, size=n, dtype=np.uint64)
+    z = rng.integers(0, 2**63, size=n, dtype=np.uint64) & rng.integers(0, 2**63, size=n, dtype=np.uint64)
+    c = rng.standard_normal(n)
+    p = lo.Params(1)
+    groups = lo.lattice_groups(8, 8)
+    t0 = time.perf_counter()
+    f = lo.features(x, z, c, groups, 100.0, 576.0)
+    lo.forward(p, f)
+    spent = time.perf_counter() - t0
+    blas = [t.get("num_threads", 1) for t in threadpool_info() if t.get("user_api") == "blas"]
+    return {"value": n / spent, "unit": "terms/s", "cores": max(blas) if blas else 1, "kind": "port",
+            "sample": f"{n} random one-word terms through the oracle's numpy features and forward, {spent:.2f} s"}
+
+
+BATCH_TOPN = 1 << 20
+
+
+def batched_line(seed: int, count: int) -> dict:
+    """The batched workload: ``count`` independent propagations (batched_instances: 16..64 qubits, depths 5..30 brick
+    layers, thresholds log-uniform in [1e-6, 1e-3], f64, Heisenberg) stepped together by pp_batch_propagate, against
+    the same instances propagated one after another.  Each instance also keeps at most BATCH_TOPN terms: without a
+    cap the deepest 64-qubit instances at the smallest thresholds grow past 10^9 terms, beyond one GPU's memory.
+    value is output terms summed over every instance's layers per second of the batched run; the per-layer counts
+    come from an untimed solo pass."""
+    from paulistrings_amd import PauliSum, circuits, propagate_batch, truncation
+
+    inst = circuits.batched_instances(seed=seed, count=count)
+    pols = [truncation.coeff(e) & truncation.topn(BATCH_TOPN) for (_, _, _, e) in inst]
+    # untimed: every instance's per-layer output counts (the merged terms the batched run produces)
+    rows_out = 0
+    layers = 0
+    for k, ((n, c, o, e), p) in enumerate(zip(inst, pols)):
+        g = PauliSum(n, *o).to_device(0)
+        st = g.propagate_with_stats(c, p, "heisenberg")
+        rows_out += int(sum(st.terms_out))
+        layers += st.layers
+        g.free()
+        if k % 16 == 15:
+            log(f"batched: counted {k + 1} of {count} instances")
+    log(f"batched: {count} instances, {layers} layers, {rows_out:.3e} output terms")
+
+    def timed(batched: bool) -> float:
+        gs = [PauliSum(n, *o).to_device(0) for (n, c, o, e) in inst]
+        gs[0].ctx.sync()
+        t0 = time.perf_counter()
+        if batched:
+            propagate_batch(gs, [c for (_, c, _, _) in inst], pols, ["heisenberg"] * len(inst))
+        else:
+            for g, (n, c, o, e), p in zip(gs, inst, pols):
+                g.propagate(c, p, "heisenberg")
+        gs[0].ctx.sync()
+        dt = time.perf_counter() - t0
+        for g in gs:
+            g.free()
+        return dt
+
+    timed(True)  # warm-up
+    t_batch = timed(True)
+    log(f"batched run {t_batch:.2f} s")
+    t_solo = timed(False)
+    log(f"one after another {t_solo:.2f} s")
+    return {
+        "metric": "merged terms/s per layer (batched)",
+        "value": rows_out / t_batch,
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": t_batch * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{count} batched propagations (16..64 qubits, depths 5..30, thresholds 1e-6..1e-3, "
+                               f"each capped at TopN 2^{BATCH_TOPN.bit_length() - 1})",
+                   "parallelism": "single GPU"},
+        "roofline": None,
+        "cpu_baseline": None,
+        "detail": {"layers": layers, "output_terms": rows_out, "seconds_batched": t_batch,
+                   "seconds_one_after_another": t_solo, "speedup": t_solo / t_batch},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +341,9 @@ def main() -> int:
     ap.add_argument("--cpu-log2n", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["saturated", "learned-score", "batched"], default="saturated",
+                    help="saturated: the headline line (default); learned-score: the learned pass's MLP scoring on "
+                         "2^22 terms; batched: 64 independent propagations stepped together")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference for the HBM "
                          "traffic counters) and exit")
@@ -210,6 +365,13 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 8, "reps": 3}), flush=True)
+        return 0
+
+    if args.workload == "learned-score":
+        print(json.dumps(learned_score_line(args.steps, args.warmup, 22)), flush=True)
+        return 0
+    if args.workload == "batched":
+        print(json.dumps(batched_line(SEED, 64)), flush=True)
         return 0
 
     channels = dense_layer_channels(SEED)

@@ -157,6 +157,9 @@ pp_status pp_expectation_product(pp_sum* sum, const uint64_t* axis_x, const uint
 pp_status pp_profile_enable(pp_context* ctx, int enable);
 pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms, uint64_t* rows_in,
                           uint64_t* rows_out);
+/* The same for the learned pass's score kernel (pp_truncate_learned, pp_propagate_learned): its launches, their
+ * total milliseconds and the terms they scored since profiling was enabled.  No counterpart in the reference. */
+pp_status pp_profile_read_score(pp_context* ctx, uint64_t* launches, double* total_ms, uint64_t* terms);
 
 /* ---- batches ------------------------------------------------------------------------------------ */
 

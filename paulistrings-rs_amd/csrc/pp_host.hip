@@ -193,6 +193,8 @@ struct pp_context {
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     uint64_t prof_rows_in = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> score_events;  // the learned pass's score kernel
+    uint64_t prof_scored = 0;
     unsigned long long* d_prof = nullptr;  // [0]: output rows of the profiled joins
 };
 
@@ -852,6 +854,10 @@ void pp_context_destroy(pp_context* ctx) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
+    for (auto& e : ctx->score_events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
     (void)hipStreamDestroy(ctx->stream);
     if (ctx->d_prof) (void)hipFree(ctx->d_prof);
     delete ctx;
@@ -1303,6 +1309,12 @@ pp_status pp_profile_enable(pp_context* ctx, int enable) {
     }
     ctx->events.clear();
     ctx->prof_rows_in = 0;
+    for (auto& e : ctx->score_events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    ctx->score_events.clear();
+    ctx->prof_scored = 0;
     if (!ctx->d_prof) {
         pp_status st = dev_alloc(ctx, (void**)&ctx->d_prof, 2 * sizeof(unsigned long long));
         if (st != PP_OK) return st;
@@ -1330,6 +1342,22 @@ pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms,
     unsigned long long out_rows = 0;
     if (ctx->d_prof) PP_HIP(ctx, hipMemcpy(&out_rows, ctx->d_prof, sizeof(out_rows), hipMemcpyDeviceToHost));
     if (rows_out) *rows_out = out_rows;
+    return PP_OK;
+}
+
+pp_status pp_profile_read_score(pp_context* ctx, uint64_t* launches, double* total_ms, uint64_t* terms) {
+    PP_NOT_CAPTURING(ctx, "pp_profile_read_score");
+    if (!ctx) return PP_ERR_INVALID_ARGUMENT;
+    PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double ms = 0.0;
+    for (auto& e : ctx->score_events) {
+        float t = 0.f;
+        PP_HIP(ctx, hipEventElapsedTime(&t, e.first, e.second));
+        ms += t;
+    }
+    if (launches) *launches = ctx->score_events.size();
+    if (total_ms) *total_ms = ms;
+    if (terms) *terms = ctx->prof_scored;
     return PP_OK;
 }
 

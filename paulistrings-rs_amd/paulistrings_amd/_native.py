@@ -114,6 +114,7 @@ def lib() -> ctypes.CDLL:
     L.pp_profile_enable.argtypes = [vp, ctypes.c_int]
     L.pp_profile_read.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64),
                                   ctypes.POINTER(u64)]
+    L.pp_profile_read_score.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)]
     L.pp_hash_rows.argtypes = [u32, u64, u32, vp, vp]
     L.pp_hash_rows.restype = i32
     for name in ("pp_context_create", "pp_sync", "pp_context_stream", "pp_sum_upload", "pp_sum_clone",
@@ -122,7 +123,7 @@ def lib() -> ctypes.CDLL:
                  "pp_graph_capture_begin", "pp_graph_capture_end", "pp_graph_launch", "pp_graph_free",
                  "pp_get_tuning", "pp_set_tuning", "pp_autotune", "pp_batch_create", "pp_batch_free",
                  "pp_batch_propagate",
-                 "pp_expectation_product", "pp_profile_enable", "pp_profile_read"):
+                 "pp_expectation_product", "pp_profile_enable", "pp_profile_read", "pp_profile_read_score"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -178,6 +179,15 @@ class Context:
         check(lib().pp_profile_read(self.handle, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(rin),
                                     ctypes.byref(rout)), self.handle)
         return n.value, ms.value, rin.value, rout.value
+
+    def profile_read_score(self):
+        """(launches, total milliseconds, terms scored) of the learned pass's score kernel since profiling began."""
+        n = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        terms = ctypes.c_uint64()
+        check(lib().pp_profile_read_score(self.handle, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(terms)),
+              self.handle)
+        return n.value, ms.value, terms.value
 
     def get_tuning(self) -> Tuning:
         t = Tuning()

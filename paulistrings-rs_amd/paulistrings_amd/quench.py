@@ -1,13 +1,14 @@
 """Learned truncation against |c| TopN on the periodic 8 x 8 transverse-field Ising quench (64 qubits, one word).
 
-    python -m paulistrings_amd.quench --h 1.0 --steps 40 --keep 65536 --judge 1048576
+    python -m paulistrings_amd.quench --h 3.0 --steps 40 --keep 2097152 4194304 8388608 --judge 67108864
 
-The average-X observable is propagated (Heisenberg) through ``steps`` Trotter steps (dt = 0.05, J = 1) three ways:
-|c| TopN(judge) as the reference value, |c| TopN(keep), and the learned pass keeping ``keep`` terms.  The network is
-trained first, on labels from plain short runs of the same lattice: terms sampled from a |c| TopN propagation of
-``train_steps`` steps, each labelled with ln|its read-out factor in that short run| (the plain engine propagating it
-alone, coefficient 1, through the layers left, threshold ``label_eps``; floored at ln(label_eps)).  Prints one JSON line: the three read-outs, the two errors against
-the judge, and each run's wall time.
+The average-X observable is propagated (Heisenberg) through ``steps`` Trotter steps (dt = 0.05, J = 1): once with
+|c| TopN(judge) as the reference value, then for every ``keep`` with |c| TopN(keep) and with the learned pass keeping
+``keep`` terms.  The network is trained first, on labels from plain short runs of the same lattice: terms sampled
+from a |c| TopN propagation of ``train_steps`` steps, each labelled with ln|its read-out factor in that short run|
+(the plain engine propagating it alone, coefficient 1, through the layers left, threshold ``label_eps``; floored at
+ln(label_eps)).  Prints one JSON line: the read-outs, their errors against the judge, each run's wall time, and each
+method's time to an error within the tolerance.
 """
 
 from __future__ import annotations
@@ -63,13 +64,17 @@ def training_set(step, train_steps: int, keep: int, per_checkpoint: int, eps: fl
     return (np.concatenate(xs), np.concatenate(zs), np.concatenate(cs), np.concatenate(rs), np.concatenate(ls))
 
 
-def run(h: float, steps: int, keep: int, judge: int, train_steps: int, per_checkpoint: int, epochs: int,
-        eps: float, label_eps: float, seed: int) -> dict:
+def run(h: float, steps: int, keeps, judge: int, train_steps: int, per_checkpoint: int, epochs: int, eps: float,
+        label_eps: float, seed: int, tolerance: float = 1e-3) -> dict:
+    """Train once, run the judge once, then |c| TopN and the learned pass at every keep in ``keeps``.  The
+    time-to-tolerance of a method is the wall time of its smallest keep whose error is within ``tolerance`` (None
+    when no keep reaches it)."""
     n = 64
+    keeps = sorted(int(k) for k in keeps)
     step = tfim_square_step(8, 8, dt=0.05, j=1.0, h=h)
     circ = repeat(step, steps)
     t0 = time.perf_counter()
-    x, z, c, rem, lab = training_set(step, train_steps, keep, per_checkpoint, eps, label_eps, seed)
+    x, z, c, rem, lab = training_set(step, train_steps, keeps[0], per_checkpoint, eps, label_eps, seed)
     t_labels = time.perf_counter() - t0
     _progress(f"{len(lab)} labels in {t_labels:.1f} s")
     # the horizon: the longest remaining depth the labels cover; deeper terms read as that depth
@@ -91,15 +96,25 @@ def run(h: float, steps: int, keep: int, judge: int, train_steps: int, per_check
         return v, dt
 
     judge_v, judge_t = timed(lambda g: g.propagate(circ, truncation.coeff(eps) & truncation.topn(judge), "heisenberg"))
-    plain_v, plain_t = timed(lambda g: g.propagate(circ, truncation.coeff(eps) & truncation.topn(keep), "heisenberg"))
-    learn_v, learn_t = timed(lambda g: propagate_learned(g, circ, model, keep, truncation.coeff(eps), "heisenberg"))
+    rows = []
+    for keep in keeps:
+        pv, pt = timed(lambda g: g.propagate(circ, truncation.coeff(eps) & truncation.topn(keep), "heisenberg"))
+        lv, lt = timed(lambda g: propagate_learned(g, circ, model, keep, truncation.coeff(eps), "heisenberg"))
+        rows.append({"keep": keep, "plain_topn": pv, "learned": lv, "err_plain": abs(pv - judge_v),
+                     "err_learned": abs(lv - judge_v), "seconds_plain": pt, "seconds_learned": lt})
     model.free()
+
+    def time_to(kind):
+        for r in rows:
+            if r["err_" + kind] <= tolerance:
+                return r["seconds_" + kind]
+        return None
+
     return {
-        "lattice": "8x8 periodic TFIM", "h": h, "dt": 0.05, "steps": steps, "layers": len(circ), "keep": keep,
-        "judge_keep": judge, "threshold": eps, "judge": judge_v, "plain_topn": plain_v, "learned": learn_v,
-        "err_plain": abs(plain_v - judge_v), "err_learned": abs(learn_v - judge_v), "seconds_judge": judge_t,
-        "seconds_plain": plain_t, "seconds_learned": learn_t, "train_samples": int(len(lab)),
-        "train_loss": loss, "seconds_labels": t_labels, "seconds_training": t_train,
+        "lattice": "8x8 periodic TFIM", "h": h, "dt": 0.05, "steps": steps, "layers": len(circ), "threshold": eps,
+        "judge_keep": judge, "judge": judge_v, "seconds_judge": judge_t, "runs": rows, "tolerance": tolerance,
+        "time_to_tolerance_plain": time_to("plain"), "time_to_tolerance_learned": time_to("learned"),
+        "train_samples": int(len(lab)), "train_loss": loss, "seconds_labels": t_labels, "seconds_training": t_train,
     }
 
 
@@ -107,7 +122,7 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--h", type=float, default=1.0)
     ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--keep", type=int, default=1 << 16)
+    ap.add_argument("--keep", type=int, nargs="+", default=[1 << 16], help="one run of each method per value")
     ap.add_argument("--judge", type=int, default=1 << 20)
     ap.add_argument("--train-steps", type=int, default=4)
     ap.add_argument("--per-checkpoint", type=int, default=2048)
@@ -115,9 +130,10 @@ def main() -> int:
     ap.add_argument("--eps", type=float, default=1e-7)
     ap.add_argument("--label-eps", type=float, default=1e-5, help="threshold of the single-term label runs")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--tolerance", type=float, default=1e-3)
     a = ap.parse_args()
-    print(json.dumps(run(a.h, a.steps, a.keep, a.judge, a.train_steps, a.per_checkpoint, a.epochs, a.eps, a.label_eps, a.seed)),
-          flush=True)
+    print(json.dumps(run(a.h, a.steps, a.keep, a.judge, a.train_steps, a.per_checkpoint, a.epochs, a.eps, a.label_eps,
+                         a.seed, a.tolerance)), flush=True)
     return 0
 
 

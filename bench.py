@@ -331,6 +331,158 @@ def batched_line(seed: int, count: int) -> dict:
     }
 
 
+def _oracle_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def tfim6x6_line(steps: int, warmup: int) -> dict:
+    """The reference's CPU-runnable case: the average-X observable on the 6 x 6 periodic TFIM (J = 1, h = 3, dt = 0.1,
+    10 Trotter steps of 72 RZZ + 36 RX), Heisenberg, CoefficientThreshold(1e-4) and weight <= 8, read out in |+>^36.
+    A step is one whole propagation plus the read-out; the oracle runs the same propagation on all host cores (at
+    most 16) and on one, and must give the same read-out."""
+    import numpy as np
+
+    import pp_oracle as po
+    from paulistrings_amd import PauliSum, truncation
+    from paulistrings_amd.circuits import repeat, tfim_square_step
+
+    n = 36
+    circ = repeat(tfim_square_step(6, 6, dt=0.1, j=1.0, h=3.0), 10)
+    x = (np.uint64(1) << np.arange(n, dtype=np.uint64)).reshape(-1, 1)
+    z = np.zeros((n, 1), np.uint64)
+    c = np.full(n, 1.0 / n)
+    pol = truncation.coeff(1e-4) & truncation.weight(8)
+    probe = PauliSum(n, x, z, c).to_device(0)
+    st = probe.propagate_with_stats(circ, pol, "heisenberg")
+    rows_out = int(sum(st.terms_out))
+    probe.free()
+    gpu_v = None
+
+    def gpu_run() -> float:
+        g = PauliSum(n, x, z, c).to_device(0)
+        g.ctx.sync()
+        t0 = time.perf_counter()
+        g.propagate(circ, pol, "heisenberg")
+        v = g.expectation("x+")
+        dt = time.perf_counter() - t0
+        g.free()
+        nonlocal gpu_v
+        gpu_v = v
+        return dt
+
+    for _ in range(warmup):
+        gpu_run()
+    gpu_s = sum(gpu_run() for _ in range(steps)) / steps
+
+    def oracle_run(threads: int):
+        o = po.OracleSum(n, x, z, c)
+        t0 = time.perf_counter()
+        o.propagate(circ.channels(), "heisenberg", po.coeff_policy(eps=1e-4, max_weight=8), threads=threads)
+        v = o.expectation("x")
+        return time.perf_counter() - t0, v
+
+    threads = _oracle_threads()
+    cpu_s, cpu_v = oracle_run(threads)
+    cpu1_s, _ = oracle_run(1)
+    log(f"tfim 6x6: gpu {gpu_s * 1e3:.1f} ms, oracle {cpu_s * 1e3:.1f} ms on {threads} threads, "
+        f"{cpu1_s * 1e3:.1f} ms on one; read-outs {gpu_v!r} / {cpu_v!r}")
+    return {
+        "metric": "merged terms/s per layer (whole propagation)",
+        "value": rows_out / gpu_s,
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": gpu_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "6x6 periodic TFIM, average X, 10 Trotter steps (1080 channels), threshold 1e-4, "
+                               "weight <= 8, Heisenberg", "parallelism": "single GPU"},
+        "roofline": None,
+        "cpu_baseline": {"value": rows_out / cpu_s, "unit": "terms/s", "cores": threads, "kind": "port",
+                         "sample": "the whole propagation (the oracle, the C restatement of the reference's host "
+                                   "engine)"},
+        "detail": {"layers": len(st.terms_out), "output_terms": rows_out, "peak_terms": st.peak_terms,
+                   "readout_gpu": gpu_v, "readout_oracle": cpu_v, "readout_abs_diff": abs(gpu_v - cpu_v),
+                   "oracle_ms_all_cores": cpu_s * 1e3, "oracle_ms_one_core": cpu1_s * 1e3,
+                   "speedup_vs_all_cores": cpu_s / gpu_s, "speedup_vs_one_core": cpu1_s / gpu_s},
+    }
+
+
+def kicked_ising_line(cpu_budget_s: float) -> dict:
+    """Kicked Ising on the 127-qubit heavy-hex graph: 20 steps of RZZ(-pi/2) on the 144 edges (greedy colouring in
+    sorted order) then RX(0.8) on every qubit, observable Z_62, f64, Heisenberg, CoefficientThreshold(1e-5) and
+    TopN(2^24), read out in |0>^127.  One untimed run counts the per-layer terms, one run is timed.  The oracle runs
+    the same circuit step by step on the host cores until cpu_budget_s is spent."""
+    import numpy as np
+
+    import pp_oracle as po
+    from paulistrings_amd import PauliSum, truncation
+    from paulistrings_amd.circuits import heavy_hex_kicked_ising
+
+    n = 127
+    circ = heavy_hex_kicked_ising(20, 0.8, -math.pi / 2, order="zz-then-x")
+    x = np.zeros((1, 2), np.uint64)
+    z = np.zeros((1, 2), np.uint64)
+    z[0, 0] = np.uint64(1) << np.uint64(62)
+    c = np.array([1.0])
+    pol = truncation.coeff(1e-5) & truncation.topn(1 << 24)
+    g = PauliSum(n, x, z, c).to_device(0)
+    st = g.propagate_with_stats(circ, pol, "heisenberg")
+    rows_out = int(sum(st.terms_out))
+    g.free()
+    log(f"kicked ising: {len(st.terms_out)} layers, peak {st.peak_terms} terms, {rows_out:.3e} output terms")
+    g = PauliSum(n, x, z, c).to_device(0)
+    g.ctx.sync()
+    t0 = time.perf_counter()
+    g.propagate(circ, pol, "heisenberg")
+    v = g.expectation("z+")
+    gpu_s = time.perf_counter() - t0
+    g.free()
+    log(f"kicked ising: {gpu_s:.2f} s, <Z62> = {v!r}")
+
+    threads = _oracle_threads()
+    per_step = len(circ) // 20
+    chans = circ.channels()
+    o = po.OracleSum(n, x, z, c)
+    opol = po.coeff_policy(eps=1e-5, topn=1 << 24)
+    spent, cpu_rows, done = 0.0, 0, 0
+    # Heisenberg order: the last step's channels first; every step is the same, so run the first `per_step` channels
+    # in Heisenberg order step after step
+    for k in range(20):
+        t1 = time.perf_counter()
+        lens = o.propagate(chans[:per_step], "heisenberg", opol, threads=threads)
+        spent += time.perf_counter() - t1
+        cpu_rows += int(lens.sum())
+        done += 1
+        if spent > cpu_budget_s:
+            break
+    return {
+        "metric": "merged terms/s per layer (whole propagation)",
+        "value": rows_out / gpu_s,
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": gpu_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "heavy-hex kicked Ising, 127 qubits, 20 steps (5420 channels), Z_62, threshold 1e-5, "
+                               "TopN 2^24, Heisenberg", "parallelism": "single GPU"},
+        "roofline": None,
+        "cpu_baseline": {"value": cpu_rows / spent, "unit": "terms/s", "cores": threads, "kind": "port",
+                         "sample": f"the oracle on the first {done} of the 20 steps ({spent:.1f} s)"},
+        "detail": {"layers": len(st.terms_out), "output_terms": rows_out, "peak_terms": st.peak_terms,
+                   "final_terms": st.final_terms, "readout": v},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -341,9 +493,11 @@ def main() -> int:
     ap.add_argument("--cpu-log2n", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["saturated", "learned-score", "batched"], default="saturated",
+    ap.add_argument("--workload", choices=["saturated", "learned-score", "batched", "tfim6x6", "kicked-ising"],
+                    default="saturated",
                     help="saturated: the headline line (default); learned-score: the learned pass's MLP scoring on "
-                         "2^22 terms; batched: 64 independent propagations stepped together")
+                         "2^22 terms; batched: 64 independent propagations stepped together; tfim6x6: the 6x6 TFIM "
+                         "against the oracle; kicked-ising: the 127-qubit heavy-hex kicked Ising with TopN 2^24")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference for the HBM "
                          "traffic counters) and exit")
@@ -372,6 +526,12 @@ def main() -> int:
         return 0
     if args.workload == "batched":
         print(json.dumps(batched_line(SEED, 64)), flush=True)
+        return 0
+    if args.workload == "tfim6x6":
+        print(json.dumps(tfim6x6_line(args.steps, args.warmup)), flush=True)
+        return 0
+    if args.workload == "kicked-ising":
+        print(json.dumps(kicked_ising_line(args.cpu_seconds)), flush=True)
         return 0
 
     channels = dense_layer_channels(SEED)

@@ -38,7 +38,7 @@ struct TopnHistArgs {
     uint32_t buckets_per_block;
     int32_t pass;
     const TopnState* state;
-    unsigned long long* hist;  // kTopnBins
+    unsigned long long* hist;  // kTopnBins, zero on entry (k_topn_select clears it after each pass)
 };
 
 template <int W, typename T>

@@ -41,35 +41,11 @@ __global__ void k_topn_init(TopnState* s, uint64_t n) {
     s->gathered = 0;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
-    __shared__ uint32_t h[kTopnBins];
-    for (int i = threadIdx.x; i < kTopnBins; i += kBlock) h[i] = 0;
-    __syncthreads();
-    const TopnState st = *a.state;
-    if (!st.keep_all_finite) {
-        const int shift = topn_shift(a.pass);
-        const uint64_t dmask = (1ull << topn_bits(a.pass)) - 1ull;
-        const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
-        const uint32_t b_hi = min(a.num_buckets, b_lo + a.buckets_per_block);
-        for (uint32_t b = b_lo; b < b_hi; ++b) {
-            const uint32_t n = a.len[b];
-            const uint64_t s = a.start[b];
-            for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
-                const uint64_t u = rank_bits<T>(a.c, a.score, s + r);
-                if ((a.score || !is_nan_bits(u)) && (u & st.mask) == st.prefix) atomicAdd(&h[(u >> shift) & dmask], 1u);
-            }
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kTopnBins; i += kBlock)
-        if (h[i]) atomicAdd(&a.hist[i], (unsigned long long)h[i]);
-}
-
-// One workgroup: pick the digit holding the k-th largest, then clear the histogram for the next pass.  The digit is
-// the largest d whose suffix count sum_{e >= d} hist[e] reaches k: each thread owns kPerThread consecutive bins, a
-// suffix scan of the per-thread sums (in LDS) finds the one thread whose bins hold d, and that thread walks them.
-__global__ void __launch_bounds__(kBlock) k_topn_select(TopnState* s, unsigned long long* hist, int pass, uint64_t n) {
+// The digit holding the k-th largest after pass `pass`, from the complete histogram, run by one workgroup; then the
+// histogram is cleared for the next pass.  The digit is the largest d whose suffix count sum_{e >= d} hist[e]
+// reaches k: each thread owns kPerThread consecutive bins, a suffix scan of the per-thread sums (in LDS) finds the
+// one thread whose bins hold d, and that thread walks them.
+__device__ __forceinline__ void select_digit(TopnState* s, unsigned long long* hist, int pass, uint64_t n) {
     constexpr int kPerThread = kTopnBins / kBlock;
     static_assert(kPerThread * kBlock == kTopnBins, "bins split evenly over the workgroup");
     __shared__ unsigned long long suffix[kBlock];
@@ -113,25 +89,63 @@ __global__ void __launch_bounds__(kBlock) k_topn_select(TopnState* s, unsigned l
         // the thread holding the digit; with no digit reaching k (not reached in a selection) thread 0 takes bin 0
         const bool holder = (above_mine < k && above_mine + sum >= k) || (t == 0 && suffix[0] < k);
         if (holder) {
-            unsigned long long acc = above_mine;
+            unsigned long long acc = above_mine, at = mine[0];
             int chosen = t * kPerThread;
+            bool found = false;
+#pragma unroll
             for (int j = kPerThread - 1; j >= 0; --j) {
-                if (acc + mine[j] >= k) {
+                if (!found && acc + mine[j] >= k) {
                     chosen = t * kPerThread + j;
-                    break;
+                    at = mine[j];
+                    found = true;
                 }
-                acc += mine[j];
+                if (!found) acc += mine[j];
             }
             const int shift = topn_shift(pass);
             s->above += acc;
             s->k -= acc;
             s->prefix |= (unsigned long long)chosen << shift;
             s->mask |= ((1ull << topn_bits(pass)) - 1ull) << shift;
-            s->ties = hist[chosen];
+            s->ties = at;
         }
     }
     __syncthreads();
     for (int i = t; i < kTopnBins; i += kBlock) hist[i] = 0;
+}
+
+// One radix pass: each workgroup histograms the digit of its buckets' terms that match the prefix found so far and
+// adds its counts to the global histogram; k_topn_select picks the digit.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
+    __shared__ uint32_t h[kTopnBins];
+    for (int i = threadIdx.x; i < kTopnBins; i += kBlock) h[i] = 0;
+    __syncthreads();
+    const TopnState st = *a.state;
+    if (!st.keep_all_finite) {
+        const int shift = topn_shift(a.pass);
+        const uint64_t dmask = (1ull << topn_bits(a.pass)) - 1ull;
+        const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
+        const uint32_t b_hi = min(a.num_buckets, b_lo + a.buckets_per_block);
+        for (uint32_t b = b_lo; b < b_hi; ++b) {
+            const uint32_t n = a.len[b];
+            const uint64_t s = a.start[b];
+            for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
+                const uint64_t u = rank_bits<T>(a.c, a.score, s + r);
+                const bool match = (a.score || !is_nan_bits(u)) && (u & st.mask) == st.prefix;
+                const uint32_t bin = (uint32_t)((u >> shift) & dmask);
+                if (match) atomicAdd(&h[bin], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTopnBins; i += kBlock)
+        if (h[i]) atomicAdd(&a.hist[i], (unsigned long long)h[i]);
+}
+
+// One workgroup after each histogram pass (a kernel boundary, not a device-scope fence per histogram workgroup: on
+// a multi-XCD device such a fence writes back the XCD's L2).
+__global__ void __launch_bounds__(kBlock) k_topn_select(TopnState* s, unsigned long long* hist, int pass, uint64_t n) {
+    select_digit(s, hist, pass, n);
 }
 
 template <int W, typename T>

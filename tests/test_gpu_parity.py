@@ -238,6 +238,28 @@ def test_approx_topn_standalone(case):
         assert np.array_equal(h.coefficients_array(), oc), (case, n)
 
 
+def test_topn_after_approx_topn_on_one_sum():
+    """ApproxTopN and TopN share the sum's histogram buffer; each selection starts from a clean one: alternating
+    passes on one resident sum keep exactly what the oracle keeps."""
+    announce("topn after approx topn on one sum")
+    rng = np.random.default_rng(37)
+    nq, m = 127, 20000
+    x, z, c = random_sum(rng, m, nq, 0.1, F64)
+    s = po.OracleSum(nq, x, z, c)
+    g = PauliSum(nq, x, z, c).to_device(0)
+    for kind, n in (("topn", 15000), ("approx", 9000), ("topn", 5000), ("approx", 3000), ("topn", 1234)):
+        getattr(s, "topn" if kind == "topn" else "approx_topn")(n)
+        if kind == "topn":
+            g.truncate_topn(n)
+        else:
+            g.truncate_approx_topn(n)
+        ox, oz, oc, _ = s.export()
+        h = g.to_host()
+        assert np.array_equal(h.x_array(), ox) and np.array_equal(h.z_array(), oz), (kind, n)
+        assert np.array_equal(h.coefficients_array(), oc), (kind, n)
+    g.free()
+
+
 @pytest.mark.parametrize("dtype", [F64, F32])
 def test_approx_topn_inside_propagation(dtype):
     announce(f"approx topn inside a propagation {dtype.__name__}")

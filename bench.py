@@ -141,13 +141,23 @@ def gpu_measure(x, z, c, channels, steps: int, warmup: int):
     }
 
 
-def cpu_baseline(channels, dtype_name: str, log2n: int, budget_s: float):
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(channels, dtype_name: str, log2n: int, budget_s: float, threads: int | None = None):
     """The oracle on this host's cores: per-channel layers on a 2^log2n sample of the same workload."""
     import numpy as np
 
     import pp_oracle as po
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
     x, z, c = saturated_keys(1 << log2n, SEED ^ 0x5A5A, dtype_name)
     base = po.OracleSum(NUM_QUBITS, x, z, c)
     pol = po.coeff_policy(eps=THRESHOLD)
@@ -547,9 +557,12 @@ def main() -> int:
     log(f"{value:.3e} merged terms/s, join kernel {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
 
     cpu = None
+    cpu1 = None
     if not args.no_cpu:
         log("cpu baseline (oracle on the host cores)")
         cpu = cpu_baseline(channels, args.dtype, args.cpu_log2n, args.cpu_seconds)
+        log("cpu baseline on one thread")
+        cpu1 = cpu_baseline(channels, args.dtype, args.cpu_log2n - 2, args.cpu_seconds / 3, threads=1)
 
     traffic = None
     if args.log2n == 24:
@@ -599,6 +612,8 @@ def main() -> int:
             "device_bucket_bits": r["device_bits"],
             "input_terms_per_s": r["rows_in"] / r["seconds"],
             "speedup_vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
+            "cpu_baseline_one_thread": cpu1,
+            "host": {"nproc": os.cpu_count(), "cpu_model": cpu_model()},
         },
     }
     print(json.dumps(out), flush=True)

@@ -50,19 +50,20 @@ __device__ __forceinline__ Key<W> load_key(const uint64_t* __restrict__ x, const
     return k;
 }
 
+// Plain stores: with the non-temporal hint the saturated join layer ran 2% slower (4.28 against 4.37 TB/s).
 template <int W>
 __device__ __forceinline__ void store_key(uint64_t* __restrict__ x, uint64_t* __restrict__ z, uint64_t cap,
                                           uint64_t i, const Key<W>& k) {
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        __builtin_nontemporal_store(k.x[w], &x[w * cap + i]);
-        __builtin_nontemporal_store(k.z[w], &z[w * cap + i]);
+        x[w * cap + i] = k.x[w];
+        z[w * cap + i] = k.z[w];
     }
 }
 
 template <typename T>
 __device__ __forceinline__ void store_coeff(T* __restrict__ c, uint64_t i, T v) {
-    __builtin_nontemporal_store(v, &c[i]);
+    c[i] = v;
 }
 
 // commutes_with (pauli_string.rs:116-123): anticommute iff popcount(x.gz ^ z.gx) is odd.

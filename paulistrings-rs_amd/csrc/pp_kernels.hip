@@ -115,6 +115,29 @@ __global__ void __launch_bounds__(kBlock) k_offsets_final(const OffsetsArgs a) {
         }
 }
 
+// The whole scan in one workgroup when the layout fits one tile (kScanTile buckets): the small sums' layers, whose
+// time is mostly launches, take one launch here instead of three.
+__global__ void __launch_bounds__(kBlock) k_offsets_one(const OffsetsArgs a) {
+    __shared__ uint64_t wave_sums[kWaves];
+    const uint32_t lo = threadIdx.x * kScanItems;
+    uint64_t sz[kScanItems];
+    uint64_t mine = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kScanItems; ++i) {
+        sz[i] = lo + i < a.out_n ? bucket_size(a, lo + i) : 0ull;
+        mine += sz[i];
+    }
+    uint64_t total;
+    uint64_t base = block_exclusive_scan_u64(mine, wave_sums, &total);
+#pragma unroll
+    for (uint32_t i = 0; i < kScanItems; ++i)
+        if (lo + i < a.out_n) {
+            a.out_start[lo + i] = base;
+            base += sz[i];
+        }
+    if (threadIdx.x == 0 && a.total) *a.total = total;
+}
+
 // Total and longest bucket of a layout, with a handful of atomics per launch (one per workgroup), and the
 // total added to a measurement counter when one is given.
 __global__ void __launch_bounds__(kBlock) k_reduce_lens(const uint32_t* __restrict__ len, uint32_t num_buckets,
@@ -779,6 +802,10 @@ hipError_t launch_bucket_offsets(hipStream_t st, const uint32_t* len, uint32_t n
     a.total = total;
     const uint32_t tiles = offsets_tiles(a.out_n);
     if (tiles == 0) return hipSuccess;
+    if (tiles == 1) {
+        hipLaunchKernelGGL(k_offsets_one, dim3(1), dim3(kBlock), 0, st, a);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_offsets_tiles, dim3(tiles), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_offsets_top, dim3(1), dim3(kBlock), 0, st, a, tiles);
     hipLaunchKernelGGL(k_offsets_final, dim3(tiles), dim3(kBlock), 0, st, a);

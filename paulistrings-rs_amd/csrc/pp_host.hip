@@ -393,12 +393,15 @@ pp_status scratch(pp_sum* s, ScratchSlot slot, size_t bytes, P** out) {
     return PP_OK;
 }
 
+// With clear_counters the scan also zeroes the sum's counters (before it writes the total into slot 3), in place of
+// clear_stats.
 pp_status offsets(pp_sum* s, const uint32_t* len, uint32_t num_buckets, uint32_t mode, uint32_t delta,
-                  uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start) {
+                  uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start, bool clear_counters = false) {
     pp_status st;
     if ((st = ensure_tiles(s, mode == 2 ? coarse_buckets : num_buckets)) != PP_OK) return st;
     PP_HIP(s->ctx, launch_bucket_offsets(s->ctx->stream, len, num_buckets, mode, delta, coarse_buckets, fold,
-                                         out_start, s->d_stats + 3, s->d_tiles));
+                                         out_start, s->d_stats + 3, s->d_tiles,
+                                         clear_counters ? s->d_stats : nullptr, kStatSlots));
     return PP_OK;
 }
 
@@ -602,9 +605,9 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
     const uint32_t B = 1u << src->device_bits;
     if ((st = ensure_columns(dst, out, 2 * src->len + (uint64_t)kSegAlign * B)) != PP_OK) return st;
     if ((st = ensure_buckets(dst, out, B)) != PP_OK) return st;
-    if ((st = clear_stats(dst)) != PP_OK) return st;
     const uint32_t delta = bucket_of_host(src, ch.gx, ch.gz, src->device_bits);
-    if ((st = offsets(dst, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start)) != PP_OK) return st;
+    // the scan also clears dst's counters
+    if ((st = offsets(dst, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start, true)) != PP_OK) return st;
     // lcap bounds the largest pair; the first launch stages scap anticommuting rows per pair, and the pairs with
     // more (rare: a generator anticommutes with a small fraction of a random bucket) go to a redo launch that
     // stages lcap.  LDS per workgroup, and so the resident workgroups per CU, follow scap.

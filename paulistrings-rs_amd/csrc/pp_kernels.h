@@ -130,11 +130,14 @@ struct ExportArgs {
     uint32_t* out_bucket;
 };
 
-// Exclusive scan of per-bucket sizes into out_start (three launches); tile_sums holds offsets_tiles(n) words.
+// Exclusive scan of per-bucket sizes into out_start (one launch for a layout of one tile, else three); tile_sums
+// holds offsets_tiles(n) words.  When clear is set, clear[0 .. clear_n) is zeroed before the total is written (a
+// join's counters, without a launch of their own).
 uint32_t offsets_tiles(uint32_t out_n);
 hipError_t launch_bucket_offsets(hipStream_t st, const uint32_t* len, uint32_t num_buckets, uint32_t mode,
                                  uint32_t delta, uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start,
-                                 unsigned long long* total, uint64_t* tile_sums);
+                                 unsigned long long* total, uint64_t* tile_sums, unsigned long long* clear = nullptr,
+                                 uint32_t clear_n = 0);
 // p[0 .. n) = v, as a kernel (capturable).
 hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, uint32_t n, unsigned long long v);
 // out[0] += total rows, out[1] = max(out[1], longest bucket); prof[0] += total rows when prof is given.

@@ -31,6 +31,8 @@ struct OffsetsArgs {
     uint64_t* out_start;
     uint64_t* tile_sums;
     unsigned long long* total;
+    unsigned long long* clear;  // when set, clear[0 .. clear_n) is zeroed before *total is written
+    uint32_t clear_n;
 };
 
 // Mode 0 packs the buckets densely (the export layout); modes 1 (a join's outputs: at most both buckets of the
@@ -92,7 +94,10 @@ __global__ void __launch_bounds__(kBlock) k_offsets_top(const OffsetsArgs a, uin
         if (t < tiles) a.tile_sums[t] = carry + ex;
         carry += total;
     }
-    if (threadIdx.x == 0 && a.total) *a.total = carry;
+    if (threadIdx.x == 0) {
+        for (uint32_t i = 0; i < a.clear_n; ++i) a.clear[i] = 0ull;
+        if (a.total) *a.total = carry;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_offsets_final(const OffsetsArgs a) {
@@ -135,7 +140,10 @@ __global__ void __launch_bounds__(kBlock) k_offsets_one(const OffsetsArgs a) {
             a.out_start[lo + i] = base;
             base += sz[i];
         }
-    if (threadIdx.x == 0 && a.total) *a.total = total;
+    if (threadIdx.x == 0) {
+        for (uint32_t i = 0; i < a.clear_n; ++i) a.clear[i] = 0ull;
+        if (a.total) *a.total = total;
+    }
 }
 
 // Total and longest bucket of a layout, with a handful of atomics per launch (one per workgroup), and the
@@ -789,8 +797,11 @@ uint32_t offsets_tiles(uint32_t out_n) { return (out_n + kScanTile - 1) / kScanT
 
 hipError_t launch_bucket_offsets(hipStream_t st, const uint32_t* len, uint32_t num_buckets, uint32_t mode,
                                  uint32_t delta, uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start,
-                                 unsigned long long* total, uint64_t* tile_sums) {
+                                 unsigned long long* total, uint64_t* tile_sums, unsigned long long* clear,
+                                 uint32_t clear_n) {
     OffsetsArgs a{};
+    a.clear = clear;
+    a.clear_n = clear ? clear_n : 0u;
     a.len = len;
     a.out_n = mode == 2 ? coarse_buckets : num_buckets;
     a.mode = mode;

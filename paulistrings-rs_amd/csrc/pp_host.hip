@@ -632,6 +632,10 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
     }
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     const bool timed = ctx->profiling && !ctx->capturing;
+    // Up to kJoinCountPairs pairs, the join's workgroups add the totals themselves (one same-address atomic each)
+    // and the length-reduce launch is skipped; measurement keeps the reduce, which also feeds the profile counter.
+    constexpr uint32_t kJoinCountPairs = 512;
+    const bool count_in_join = !timed && cosets <= kJoinCountPairs;
     if (timed) {
         PP_HIP(ctx, hipEventCreate(&ev0));
         PP_HIP(ctx, hipEventCreate(&ev1));
@@ -672,6 +676,7 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         a.stats = dst->d_stats;
         a.redo = dst->d_redo;
         a.redo_count = dst->d_stats + 4;
+        a.count_totals = count_in_join ? 1u : 0u;
         hipError_t le = launch_rotation_join<W, T>(ctx->stream, a, ctx->tuning.groups_per_cu, ctx->tuning.join_block);
         if (le != hipSuccess || scap >= lcap) return le;  // a first launch staging lcap never queues a pair
         a.redo_mode = 1;
@@ -685,7 +690,8 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         ctx->events.push_back({ev0, ev1});
         ctx->prof_rows_in += src->len;
     }
-    PP_HIP(ctx, launch_reduce_lens(ctx->stream, dst->bk[out].len, B, dst->d_stats, timed ? ctx->d_prof : nullptr));
+    if (!count_in_join)
+        PP_HIP(ctx, launch_reduce_lens(ctx->stream, dst->bk[out].len, B, dst->d_stats, timed ? ctx->d_prof : nullptr));
     return PP_OK;
 }
 

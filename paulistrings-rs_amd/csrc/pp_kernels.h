@@ -41,6 +41,8 @@ struct JoinArgs {
     uint32_t* redo;                  // pairs the first launch could not stage, for the redo launch
     unsigned long long* redo_count;  // their number
     uint32_t redo_mode;              // 0: every pair; 1: the queued pairs, staging up to scap = the largest pair
+    uint32_t count_totals;           // each pair adds its rows to stats[0] and maxes stats[1] (few buckets: no
+                                     // k_reduce_lens launch)
 };
 
 template <int W, typename T>

@@ -514,6 +514,10 @@ __device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, 
                 if (threadIdx.x == 0) {
                     a.olen[m.b0] = cur0;
                     if (pair) a.olen[m.b1] = cur1;
+                    if (a.count_totals) {  // few pairs: these atomics cost less than a reduce launch
+                        atomicAdd(&a.stats[0], (unsigned long long)(cur0 + cur1));
+                        atomicMax(&a.stats[1], (unsigned long long)max(cur0, cur1));
+                    }
                 }
             }
             if (!have_next) break;

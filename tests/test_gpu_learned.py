@@ -110,3 +110,49 @@ def test_learned_pass_keeps_the_top_scores(n):
     m.free()
     assert np.array_equal(got.x_array()[:, 0], hx[keep]) and np.array_equal(got.z_array()[:, 0], hz[keep])
     assert np.array_equal(got.coefficients_array(), hc[keep])
+
+
+def test_propagate_learned_matches_a_layer_by_layer_emulation():
+    """pp_propagate_learned against the oracle stepped one channel at a time, with the learned pass emulated on the
+    host: after every layer that leaves more than `keep` terms, keep the `keep` of largest score, the score being the
+    device network's prediction for the oracle's terms (pp_model_predict, the same per-term arithmetic as the pass's
+    score kernel) plus ln|c| in fp32.  While a run's worst-case growth stays within `keep` the device joins layers
+    into runs without a pass, and then no layer of the run exceeds `keep` either, so the two agree layer by layer.
+    Kept keys and coefficients must be equal bit for bit."""
+    import pp_oracle as po
+    from paulistrings_amd import truncation
+    from paulistrings_amd.circuits import repeat, tfim_square_step
+
+    announce("propagate_learned against a layer-by-layer emulation")
+    n_q, keep, eps = 16, 300, 1e-7
+    circ = repeat(tfim_square_step(4, 4, dt=0.1, j=1.0, h=2.0), 2)
+    chans = circ.channels()
+    m = PL.Model(PL.lattice_groups(4, 4), depth_scale=48.0, seed=21)
+    x0 = (np.uint64(1) << np.arange(n_q, dtype=np.uint64)).reshape(-1, 1)
+    z0 = np.zeros((n_q, 1), np.uint64)
+    c0 = np.full(n_q, 1.0 / n_q)
+
+    g = PauliSum(n_q, x0, z0, c0).to_device(0)
+    PL.propagate_learned(g, circ, m, keep, truncation.coeff(eps), "heisenberg")
+    got = g.to_host()
+    g.free()
+
+    x, z, c = x0.copy(), z0.copy(), c0.copy()
+    passes = 0
+    for k, j in enumerate(range(len(chans) - 1, -1, -1)):  # Heisenberg order: the last channel first
+        o = po.OracleSum(n_q, x, z, c)
+        o.propagate(chans[j:j + 1], "heisenberg", po.coeff_policy(eps=eps))
+        x, z, c, _ = o.export()
+        if len(c) > keep:
+            remaining = np.float32(len(chans) - (k + 1))
+            score = m.predict(x[:, 0], z[:, 0], c, remaining) + np.log(np.abs(c)).astype(np.float32)
+            order = np.argsort(-score.astype(np.float64), kind="stable")  # ties: the canonical order
+            kept = np.sort(order[:keep])
+            x, z, c = x[kept], z[kept], c[kept]
+            passes += 1
+    m.free()
+    assert passes > 10  # the pass did decide most layers
+    got_terms = {(int(a), int(b)): float(v) for a, b, v in zip(got.x_array()[:, 0], got.z_array()[:, 0],
+                                                               got.coefficients_array())}
+    want_terms = {(int(a), int(b)): float(v) for a, b, v in zip(x[:, 0], z[:, 0], c)}
+    assert got_terms == want_terms  # same keys, coefficients equal as doubles

@@ -4,10 +4,10 @@
 
 The average-X observable is propagated (Heisenberg) through ``steps`` Trotter steps (dt = 0.05, J = 1): once with
 |c| TopN(judge) as the reference value, then for every ``keep`` with |c| TopN(keep) and with the learned pass keeping
-``keep`` terms.  The network is trained first, on labels from plain short runs of the same lattice: terms sampled
-from a |c| TopN propagation of ``train_steps`` steps, each labelled with ln|its read-out factor in that short run|
-(the plain engine propagating it alone, coefficient 1, through the layers left, threshold ``label_eps``; floored at
-ln(label_eps)).  Prints one JSON line: the read-outs, their errors against the judge, each run's wall time, and each
+``keep`` terms.  The network is trained first, on labels from plain short runs of the 8 x 8 lattice and of the 6 x 6
+one at the same field (its keys embedded in the 8 x 8 sites): terms sampled from a |c| TopN propagation of
+``train_steps`` steps, each labelled with ln|its read-out factor in that short run| (the plain engine propagating it
+alone, coefficient 1, through the layers left, threshold ``label_eps``; floored at ln(label_eps)).  Prints one JSON line: the read-outs, their errors against the judge, each run's wall time, and each
 method's time to an error within the tolerance.
 """
 
@@ -33,6 +33,18 @@ def _progress(msg: str) -> None:
 def average_x(n: int) -> PauliSum:
     x = (np.uint64(1) << np.arange(n, dtype=np.uint64)).reshape(-1, 1)
     return PauliSum(n, x, np.zeros((n, 1), np.uint64), np.full(n, 1.0 / n))
+
+
+def embed_in_8x8(words: np.ndarray, lx: int, ly: int) -> np.ndarray:
+    """Keys of an lx-by-ly lattice (qubit y * lx + x) moved to the same sites of the 8 x 8 lattice (qubit y * 8 + x),
+    so that the 8 x 8 row and column groups count the smaller lattice's rows and columns."""
+    words = np.asarray(words, np.uint64)
+    out = np.zeros_like(words)
+    for y in range(ly):
+        for x in range(lx):
+            bit = (words >> np.uint64(y * lx + x)) & np.uint64(1)
+            out |= bit << np.uint64(y * 8 + x)
+    return out
 
 
 def training_set(step, train_steps: int, keep: int, per_checkpoint: int, eps: float, label_eps: float, seed: int):
@@ -75,8 +87,17 @@ def run(h: float, steps: int, keeps, judge: int, train_steps: int, per_checkpoin
     circ = repeat(step, steps)
     t0 = time.perf_counter()
     x, z, c, rem, lab = training_set(step, train_steps, keeps[0], per_checkpoint, eps, label_eps, seed)
+    # and from the 6 x 6 lattice at the same field, its keys embedded in the 8 x 8 sites and its remaining layers
+    # counted in 8 x 8 steps (192 channels a step against 108)
+    step6 = tfim_square_step(6, 6, dt=0.05, j=1.0, h=h)
+    x6, z6, c6, rem6, lab6 = training_set(step6, train_steps, keeps[0], per_checkpoint, eps, label_eps, seed + 1)
+    x = np.concatenate([x, embed_in_8x8(x6, 6, 6)])
+    z = np.concatenate([z, embed_in_8x8(z6, 6, 6)])
+    c = np.concatenate([c, c6])
+    rem = np.concatenate([rem, (rem6 * (len(step) / len(step6))).astype(np.float32)])
+    lab = np.concatenate([lab, lab6])
     t_labels = time.perf_counter() - t0
-    _progress(f"{len(lab)} labels in {t_labels:.1f} s")
+    _progress(f"{len(lab)} labels (8 x 8 and 6 x 6) in {t_labels:.1f} s")
     # the horizon: the longest remaining depth the labels cover; deeper terms read as that depth
     model = Model(lattice_groups(8, 8), depth_scale=float(max(1, (train_steps - 1) * len(step))), seed=seed)
     t0 = time.perf_counter()

@@ -11,8 +11,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pp_hip.h"
@@ -86,6 +88,60 @@ uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
+}
+
+// Host threads for a host-side pass over `items` elements: up to 16, one per 2^16 items.
+unsigned host_threads(uint64_t items) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(hw, items >> 16));
+}
+
+// fn(t, T) on T host threads (the calling thread runs t = 0).
+template <class F>
+void host_parallel(unsigned T, F fn) {
+    std::vector<std::thread> pool;
+    pool.reserve(T > 0 ? T - 1 : 0);
+    for (unsigned t = 1; t < T; ++t) pool.emplace_back(fn, t, T);
+    fn(0u, std::max(T, 1u));
+    for (auto& th : pool) th.join();
+}
+
+// [lo, hi) of part t of T equal parts of n.
+inline void part_range(uint64_t n, unsigned t, unsigned T, uint64_t& lo, uint64_t& hi) {
+    lo = n * t / T;
+    hi = n * (t + 1) / T;
+}
+
+// Stable counting sort of the indices 0..n-1 by key[i] < K on T threads: order[] lists the indices bucket by bucket,
+// each bucket in index order (each thread scatters a contiguous index range after the threads before it), and
+// start[k] .. start[k + 1] is bucket k's range.
+void parallel_counting_sort(const uint32_t* key, uint64_t n, uint32_t K, unsigned T, std::vector<uint64_t>& order,
+                            std::vector<uint64_t>& start) {
+    std::vector<uint64_t> cnt((size_t)T * K, 0);
+    host_parallel(T, [&](unsigned t, unsigned TT) {
+        uint64_t lo, hi;
+        part_range(n, t, TT, lo, hi);
+        uint64_t* c = &cnt[(size_t)t * K];
+        for (uint64_t i = lo; i < hi; ++i) ++c[key[i]];
+    });
+    start.assign((size_t)K + 1, 0);
+    uint64_t acc = 0;
+    for (uint32_t k = 0; k < K; ++k) {
+        start[k] = acc;
+        for (unsigned t = 0; t < T; ++t) {
+            const uint64_t c = cnt[(size_t)t * K + k];
+            cnt[(size_t)t * K + k] = acc;  // thread t's first slot in bucket k
+            acc += c;
+        }
+    }
+    start[K] = acc;
+    order.resize(n);
+    host_parallel(T, [&](unsigned t, unsigned TT) {
+        uint64_t lo, hi;
+        part_range(n, t, TT, lo, hi);
+        uint64_t* c = &cnt[(size_t)t * K];
+        for (uint64_t i = lo; i < hi; ++i) order[c[key[i]]++] = i;
+    });
 }
 
 uint64_t live_mask(uint32_t num_qubits, uint32_t word) {
@@ -947,41 +1003,58 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
     while (dbits < s->max_bits && m > target * (1ull << dbits)) ++dbits;
     s->device_bits = dbits;
     const uint32_t B = 1u << dbits;
-    std::vector<uint32_t> bucket(m), counts(B, 0);
-    for (uint64_t t = 0; t < m; ++t) {
-        const uint64_t i = keep_idx[t];
-        bucket[t] = bucket_of_host(s, x + i * width, z + i * width, dbits);
-        counts[bucket[t]]++;
-    }
+    // The bucket of every kept term, then a stable counting sort by bucket, on up to 16 host threads (the same
+    // layout as one thread would make).
+    const unsigned T = host_threads(m);
+    std::vector<uint32_t> bucket(m);
+    host_parallel(T, [&](unsigned t, unsigned TT) {
+        uint64_t lo, hi;
+        part_range(m, t, TT, lo, hi);
+        for (uint64_t k = lo; k < hi; ++k) {
+            const uint64_t i = keep_idx[k];
+            bucket[k] = bucket_of_host(s, x + i * width, z + i * width, dbits);
+        }
+    });
+    std::vector<uint64_t> order, bstart;
+    parallel_counting_sort(bucket.data(), m, B, T, order, bstart);
     std::vector<uint64_t> start(B);
     std::vector<uint32_t> lens(B);
     uint64_t acc = 0;
     uint32_t maxb = 0;
     for (uint32_t b = 0; b < B; ++b) {
+        const uint32_t cnt = (uint32_t)(bstart[b + 1] - bstart[b]);
         start[b] = acc;
-        lens[b] = counts[b];
-        acc += (counts[b] + (kSegAlign - 1)) & ~(kSegAlign - 1);
-        maxb = std::max(maxb, counts[b]);
+        lens[b] = cnt;
+        acc += (cnt + (kSegAlign - 1)) & ~(kSegAlign - 1);
+        maxb = std::max(maxb, cnt);
     }
     s->max_bucket = maxb;
     if ((st = ensure_columns(s, 0, std::max<uint64_t>(acc, 1))) != PP_OK) return bail(st);
     if ((st = ensure_buckets(s, 0, B)) != PP_OK) return bail(st);
     const uint64_t cap = s->col[0].cap;
-    std::vector<uint64_t> hx((size_t)width * cap, 0), hz((size_t)width * cap, 0);
-    std::vector<unsigned char> hc(cap * cb, 0);
-    std::vector<uint64_t> cursor(start);
-    for (uint64_t t = 0; t < m; ++t) {
-        const uint64_t i = keep_idx[t];
-        const uint64_t pos = cursor[bucket[t]]++;
-        for (uint32_t w = 0; w < width; ++w) {
-            hx[w * cap + pos] = x[i * width + w];
-            hz[w * cap + pos] = z[i * width + w];
-        }
-        std::memcpy(&hc[pos * cb], (const unsigned char*)coeff + i * cb, cb);
-    }
-    if (hipMemcpy(s->col[0].x, hx.data(), hx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->col[0].z, hz.data(), hz.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->col[0].c, hc.data(), hc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+    // host staging of the word-major columns, left uninitialised: the rows between segments are never read
+    std::unique_ptr<uint64_t[]> hx(new (std::nothrow) uint64_t[(size_t)width * cap]);
+    std::unique_ptr<uint64_t[]> hz(new (std::nothrow) uint64_t[(size_t)width * cap]);
+    std::unique_ptr<unsigned char[]> hc(new (std::nothrow) unsigned char[cap * cb]);
+    if (!hx || !hz || !hc) return bail(fail(ctx, PP_ERR_OUT_OF_MEMORY, "host staging allocation"));
+    // word-major columns: bucket b's k-th term (in input order) at start[b] + k
+    host_parallel(T, [&](unsigned t, unsigned TT) {
+        uint64_t lo, hi;
+        part_range(B, t, TT, lo, hi);
+        for (uint64_t b = lo; b < hi; ++b)
+            for (uint64_t r = bstart[b]; r < bstart[b + 1]; ++r) {
+                const uint64_t i = keep_idx[order[r]];
+                const uint64_t pos = start[b] + (r - bstart[b]);
+                for (uint32_t w = 0; w < width; ++w) {
+                    hx[w * cap + pos] = x[i * width + w];
+                    hz[w * cap + pos] = z[i * width + w];
+                }
+                std::memcpy(&hc[pos * cb], (const unsigned char*)coeff + i * cb, cb);
+            }
+    });
+    if (hipMemcpy(s->col[0].x, hx.get(), (size_t)width * cap * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->col[0].z, hz.get(), (size_t)width * cap * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->col[0].c, hc.get(), cap * cb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->bk[0].start, start.data(), B * 8, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->bk[0].len, lens.data(), B * 4, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(ctx, PP_ERR_DRIVER, "upload copy failed"));
@@ -1139,23 +1212,27 @@ pp_status pp_sum_download(pp_sum* s, uint64_t* x, uint64_t* z, void* coeff) {
             a.out_bucket = db;
             return launch_export<WW, T>(ctx->stream, a, B);
         });
-    std::vector<uint64_t> hx(n * W), hz(n * W);
-    std::vector<unsigned char> hc(n * cb);
-    std::vector<uint32_t> hb(n);
-    if (e == hipSuccess) e = hipMemcpyAsync(hx.data(), dx, n * W * 8, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(hz.data(), dz, n * W * 8, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(hc.data(), dc, n * cb, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(hb.data(), db, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    // host staging, left uninitialised: the copies fill every element
+    std::unique_ptr<uint64_t[]> hx(new (std::nothrow) uint64_t[n * W]), hz(new (std::nothrow) uint64_t[n * W]);
+    std::unique_ptr<unsigned char[]> hc(new (std::nothrow) unsigned char[n * cb]);
+    std::unique_ptr<uint32_t[]> hb(new (std::nothrow) uint32_t[n]);
+    if (!hx || !hz || !hc || !hb) {
+        cleanup();
+        return fail(ctx, PP_ERR_OUT_OF_MEMORY, "host staging allocation");
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(hx.get(), dx, n * W * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hz.get(), dz, n * W * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hc.get(), dc, n * cb, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hb.get(), db, n * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     cleanup();
     if (e != hipSuccess) return fail(ctx, PP_ERR_DRIVER, std::string("download: ") + hipGetErrorString(e));
-    // canonical order: the reference's bucket ascending (counting sort), then lexicographic key per bucket
+    // canonical order: the reference's bucket ascending (counting sort), then lexicographic key per bucket; each
+    // step on up to 16 host threads (keys are distinct, so the order is the same however the work is split)
     const uint32_t RB = 1u << s->reference_bits;
-    std::vector<uint64_t> rstart(RB + 1, 0);
-    for (uint64_t i = 0; i < n; ++i) rstart[hb[i] + 1]++;
-    for (uint32_t b = 0; b < RB; ++b) rstart[b + 1] += rstart[b];
-    std::vector<uint64_t> order(n), cursor(rstart.begin(), rstart.end() - 1);
-    for (uint64_t i = 0; i < n; ++i) order[cursor[hb[i]]++] = i;
+    const unsigned T = host_threads(n);
+    std::vector<uint64_t> order, rstart;
+    parallel_counting_sort(hb.get(), n, RB, T, order, rstart);
     auto less = [&](uint64_t a, uint64_t b) {
         for (uint32_t w = 0; w < W; ++w)
             if (hx[a * W + w] != hx[b * W + w]) return hx[a * W + w] < hx[b * W + w];
@@ -1163,15 +1240,23 @@ pp_status pp_sum_download(pp_sum* s, uint64_t* x, uint64_t* z, void* coeff) {
             if (hz[a * W + w] != hz[b * W + w]) return hz[a * W + w] < hz[b * W + w];
         return false;
     };
-    for (uint32_t b = 0; b < RB; ++b) std::sort(order.begin() + rstart[b], order.begin() + rstart[b + 1], less);
-    for (uint64_t t = 0; t < n; ++t) {
-        const uint64_t i = order[t];
-        for (uint32_t w = 0; w < W; ++w) {
-            x[t * W + w] = hx[i * W + w];
-            z[t * W + w] = hz[i * W + w];
+    host_parallel(T, [&](unsigned t, unsigned TT) {
+        uint64_t lo, hi;
+        part_range(RB, t, TT, lo, hi);
+        for (uint64_t b = lo; b < hi; ++b) std::sort(order.begin() + rstart[b], order.begin() + rstart[b + 1], less);
+    });
+    host_parallel(T, [&](unsigned t, unsigned TT) {
+        uint64_t lo, hi;
+        part_range(n, t, TT, lo, hi);
+        for (uint64_t k = lo; k < hi; ++k) {
+            const uint64_t i = order[k];
+            for (uint32_t w = 0; w < W; ++w) {
+                x[k * W + w] = hx[i * W + w];
+                z[k * W + w] = hz[i * W + w];
+            }
+            std::memcpy((unsigned char*)coeff + k * cb, &hc[i * cb], cb);
         }
-        std::memcpy((unsigned char*)coeff + t * cb, &hc[i * cb], cb);
-    }
+    });
     return PP_OK;
 }
 

@@ -267,7 +267,7 @@ struct BucketArrays {
     uint64_t cap = 0;
 };
 
-enum ScratchSlot { kScrKeyX, kScrKeyZ, kScrPos, kScrBucket, kScrFlags, kScrCounts, kScrGathered, kScratchSlots };
+enum ScratchSlot { kScrKeyX, kScrKeyZ, kScrPos, kScrBucket, kScrFlags, kScrCounts, kScrGathered, kScrRank, kScratchSlots };
 
 struct pp_sum {
     pp_context* ctx = nullptr;

@@ -57,6 +57,7 @@ struct TopnTieArgs {
     uint64_t* tie_z;
     uint32_t* tie_bucket;
     uint64_t* tie_pos;
+    uint64_t* tie_rank;  // the rank key of each gathered term (the boundary bin's terms need not be equal)
     uint64_t tie_cap;
 };
 

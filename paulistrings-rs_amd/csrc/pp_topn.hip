@@ -153,10 +153,12 @@ __global__ void __launch_bounds__(kBlock) k_topn_gather_ties(const TopnTieArgs<W
     const uint32_t b = blockIdx.x;
     const uint32_t n = a.len[b];
     const uint64_t s = a.start[b];
-    const uint64_t t = a.state->prefix;
+    // the terms of the boundary bin: rank key equal to the prefix on the decided bits (a NaN never matches a
+    // finite prefix: its exponent bits are all ones)
+    const uint64_t t = a.state->prefix, mask = a.state->mask;
     for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
         const uint64_t u = rank_bits<T>(a.c, a.score, s + r);
-        if (u != t) continue;
+        if ((u & mask) != t || (!a.score && is_nan_bits(u))) continue;
         const uint64_t slot = atomicAdd(&a.state->gathered, 1ull);
         if (slot >= a.tie_cap) continue;
         const Key<W> k = load_key<W>(a.x, a.z, a.cap, s + r);
@@ -167,6 +169,7 @@ __global__ void __launch_bounds__(kBlock) k_topn_gather_ties(const TopnTieArgs<W
         }
         a.tie_bucket[slot] = bucket_of<W>(k, a.rows, a.ref_bits);
         a.tie_pos[slot] = s + r;
+        a.tie_rank[slot] = u;
     }
 }
 
@@ -188,10 +191,13 @@ __global__ void __launch_bounds__(kBlock) k_topn_retain(const TopnRetainArgs<W, 
             c = a.c[s + r];
             const uint64_t u = rank_bits<T>(a.c, a.score, s + r);
             const bool nan = !a.score && is_nan_bits(u);
+            // above the boundary bin on the decided bits: kept; in it: kept when flagged (or, without flags,
+            // every one of them)
+            const uint64_t um = u & st.mask;
             if (st.keep_all_finite) keep = !nan;
             else if (nan) keep = false;
-            else if (u > st.prefix) keep = true;
-            else if (u == st.prefix) keep = a.tie_keep == nullptr || a.tie_keep[s + r] != 0;
+            else if (um > st.prefix) keep = true;
+            else if (um == st.prefix) keep = a.tie_keep == nullptr || a.tie_keep[s + r] != 0;
         }
         bool f[1] = {keep};
         uint32_t off[1], tot[1];

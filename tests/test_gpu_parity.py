@@ -202,6 +202,34 @@ def test_topn_ties_in_canonical_order(n):
     assert np.array_equal(h.coefficients_array(), oc)
 
 
+@pytest.mark.parametrize("case", ["near_w1", "near_w2", "crowded_w1"])
+def test_topn_boundary_bin_ranked_by_magnitude_then_canonical_order(case):
+    """TopN decides the top 33 bits of |c|^2 on the device and ranks the boundary bin's terms on the host: here the
+    bin holds |c|^2 values that differ only in the low mantissa bits, with exact repeats among them.  In the crowded
+    case the bin holds over 2^16 terms, which the device narrows to equal |c|^2 before ranking."""
+    announce(f"topn boundary bin {case}")
+    rng = np.random.default_rng(41)
+    nq, m = (40, 150000) if case == "crowded_w1" else (100 if case == "near_w2" else 40, 20000)
+    x, z, _ = random_sum(rng, m, nq, 0.2, F64)
+    if case == "crowded_w1":
+        steps = rng.integers(0, 3, size=m)  # three |c| values 2^-40 apart, 50000 terms each
+    else:
+        steps = rng.integers(0, 400, size=m)  # 400 values 2^-40 apart, 50 terms each, in one 33-bit bin
+    c = 0.5 * (1.0 + steps * 2.0 ** -40) * rng.choice([-1.0, 1.0], size=m)
+    far = rng.random(m) < 0.3
+    c[far] = rng.uniform(-1.0, 1.0, size=int(far.sum()))
+    for n in (1, 777, m // 2, m - 7):
+        s = po.OracleSum(nq, x, z, c)
+        s.topn(n)
+        ox, oz, oc, _ = s.export()
+        g = PauliSum(nq, x, z, c).to_device(0)
+        g.truncate_topn(n)
+        h = g.to_host()
+        g.free()
+        assert np.array_equal(h.x_array(), ox) and np.array_equal(h.z_array(), oz), (case, n)
+        assert np.array_equal(h.coefficients_array(), oc), (case, n)
+
+
 def test_topn_inside_propagation():
     announce("topn inside a propagation")
     rng = np.random.default_rng(9)

@@ -119,8 +119,11 @@ pp_status pp_sum_bucket_bits(pp_sum* sum, int32_t* reference_bits, int32_t* devi
 
 /* propagate (engine/mod.rs:121-187 -> engine/driver.rs:319-592): every channel in circuit order
  * (reversed, each adjoint, for PP_HEISENBERG), each layer's outputs kept iff nonzero and the policy's
- * per-term filter passes, the TopN pass after every layer.  layer_len (optional, n_channels entries)
- * receives the length after every layer.  options may be NULL for the defaults. */
+ * per-term filter passes, the TopN pass after every run of layers (Runs::take).  A run of one-to-one steps
+ * (engine/bucketed/permutation_run.rs) is walked in one pass with the same keys, coefficients and per-layer
+ * lengths as layer by layer (PP_PERMUTATION_RUNS=layers in the environment selects the latter).  layer_len
+ * (optional, n_channels entries) receives the length after every layer.  options may be NULL for the
+ * defaults. */
 pp_status pp_propagate(pp_sum* sum, const pp_channel* channels, uint64_t n_channels, pp_direction direction,
                        const pp_policy* policy, const pp_options* options, uint64_t* layer_len);
 

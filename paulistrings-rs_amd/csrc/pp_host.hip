@@ -267,7 +267,21 @@ struct BucketArrays {
     uint64_t cap = 0;
 };
 
-enum ScratchSlot { kScrKeyX, kScrKeyZ, kScrPos, kScrBucket, kScrFlags, kScrCounts, kScrGathered, kScrRank, kScratchSlots };
+enum ScratchSlot {
+    kScrKeyX,
+    kScrKeyZ,
+    kScrPos,
+    kScrBucket,
+    kScrFlags,
+    kScrCounts,
+    kScrGathered,
+    kScrRank,
+    kScrDest,        // a permutation run's per-row output bucket
+    kScrRunCounts,   // its per-bucket destination counts
+    kScrDeaths,      // its per-step drops
+    kScrSteps,       // its steps
+    kScratchSlots
+};
 
 struct pp_sum {
     pp_context* ctx = nullptr;
@@ -790,6 +804,103 @@ pp_status device_layer(pp_sum* s, const pp_channel& ch, const Prepared& p, const
     return PP_OK;
 }
 
+// Whether pp_propagate walks permutation runs in one pass (default) or layer by layer (PP_PERMUTATION_RUNS=layers,
+// for comparison: both give the same keys and coefficients).
+bool fused_permutation_runs() {
+    const char* e = std::getenv("PP_PERMUTATION_RUNS");
+    return !(e && std::strcmp(e, "layers") == 0);
+}
+
+// A permutation run on the device (engine/bucketed/permutation_run.rs:297-...): k_perm_walk takes every row through
+// the steps in registers, in place, counting each output bucket and each step's drops; an offset scan lays out the
+// destinations; k_perm_scatter moves the rows into the spare columns.  len_after[j] receives the length after step
+// j, as the per-layer path would report it.
+pp_status permutation_run_device(pp_sum* s, const PermStep* steps, uint32_t n_steps, const KeepProgram& keep,
+                                 uint64_t* len_after) {
+    pp_context* ctx = s->ctx;
+    pp_status st;
+    if ((st = settle_device_bits(s)) != PP_OK) return st;
+    const uint32_t B = 1u << s->device_bits;
+    uint32_t *d_dest = nullptr, *d_counts = nullptr;
+    unsigned long long* d_deaths = nullptr;
+    PermStep* d_steps = nullptr;
+    if ((st = scratch(s, kScrRunCounts, (size_t)B * 4, &d_counts)) != PP_OK ||
+        (st = scratch(s, kScrDeaths, (size_t)kMaxPermSteps * 8, &d_deaths)) != PP_OK ||
+        (st = scratch(s, kScrSteps, (size_t)kMaxPermSteps * sizeof(PermStep), &d_steps)) != PP_OK)
+        return st;
+    const uint64_t len_in = s->len;
+    std::vector<unsigned long long> deaths(n_steps, 0);
+    for (uint32_t first = 0; first < n_steps; first += kMaxPermSteps) {
+        const uint32_t m = std::min(kMaxPermSteps, n_steps - first);
+        const int in = s->cur, out = 1 - s->cur;
+        if ((st = scratch(s, kScrDest, s->col[in].cap * 4, &d_dest)) != PP_OK) return st;
+        // every step's device bucket delta under the current bucketing
+        std::vector<PermStep> chunk(steps + first, steps + first + m);
+        for (PermStep& p : chunk) {
+            uint64_t gx[4] = {p.gx[0], p.gx[1], 0, 0}, gz[4] = {p.gz[0], p.gz[1], 0, 0};
+            p.delta = bucket_of_host(s, gx, gz, s->device_bits);
+        }
+        PP_HIP(ctx, hipMemcpyAsync(d_steps, chunk.data(), m * sizeof(PermStep), hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(ctx, hipMemsetAsync(d_counts, 0, (size_t)B * 4, ctx->stream));
+        PP_HIP(ctx, hipMemsetAsync(d_deaths, 0, (size_t)m * 8, ctx->stream));
+        hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
+            constexpr int W = decltype(Wc)::value;
+            using T = decltype(Tv);
+            PermWalkArgs<W, T> a{};
+            a.x = s->col[in].x;
+            a.z = s->col[in].z;
+            a.c = (T*)s->col[in].c;
+            a.cap = s->col[in].cap;
+            a.start = s->bk[in].start;
+            a.len = s->bk[in].len;
+            a.steps = d_steps;
+            a.num_steps = m;
+            a.keep = keep;
+            a.dest = d_dest;
+            a.counts = d_counts;
+            a.deaths = d_deaths;
+            return launch_perm_walk<W, T>(ctx->stream, a, B);
+        });
+        PP_HIP(ctx, e);
+        if ((st = ensure_columns(s, out, s->len + (uint64_t)kSegAlign * B)) != PP_OK) return st;
+        if ((st = ensure_buckets(s, out, B)) != PP_OK) return st;
+        if ((st = offsets(s, d_counts, B, 3, 0, 0, 0, s->bk[out].start)) != PP_OK) return st;
+        PP_HIP(ctx, hipMemsetAsync(s->bk[out].len, 0, (size_t)B * 4, ctx->stream));
+        e = dispatch(s, [&](auto Wc, auto Tv) {
+            constexpr int W = decltype(Wc)::value;
+            using T = decltype(Tv);
+            PermScatterArgs<W, T> a{};
+            a.x = s->col[in].x;
+            a.z = s->col[in].z;
+            a.c = (const T*)s->col[in].c;
+            a.cap = s->col[in].cap;
+            a.start = s->bk[in].start;
+            a.len = s->bk[in].len;
+            a.dest = d_dest;
+            a.ox = s->col[out].x;
+            a.oz = s->col[out].z;
+            a.oc = (T*)s->col[out].c;
+            a.ocap = s->col[out].cap;
+            a.ostart = s->bk[out].start;
+            a.olen = s->bk[out].len;
+            return launch_perm_scatter<W, T>(ctx->stream, a, B);
+        });
+        PP_HIP(ctx, e);
+        PP_HIP(ctx, hipMemcpyAsync(deaths.data() + first, d_deaths, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+        s->cur = out;
+        if ((st = count_layout(s, s->cur, B)) != PP_OK) return st;  // synchronizes
+    }
+    uint64_t len = len_in;
+    for (uint32_t j = 0; j < n_steps; ++j) {
+        len -= deaths[j];
+        if (len_after) len_after[j] = len;
+    }
+    if (len != s->len)
+        return fail(ctx, PP_ERR_DRIVER, "permutation run: " + std::to_string(s->len) + " rows placed, " +
+                                            std::to_string(len) + " survived the walk");
+    return PP_OK;
+}
+
 // Bring len and max_bucket up to date after an asynchronous out-of-place layer.
 pp_status refresh(pp_sum* s) {
     if (!s->stale) return PP_OK;
@@ -873,6 +984,41 @@ struct RunPlanner {
             if (j > i + 1) end = j;
         }
         return end;
+    }
+
+    // Layers i .. end - 1 (one run), the sum's length after each into len_after[0 .. end - i) when given.  A
+    // permutation run of two or more steps is walked in one pass; any other run layer by layer.
+    pp_status apply_run(pp_sum* s, uint64_t i, uint64_t end, const KeepProgram& keep, uint64_t* len_after) const {
+        pp_status st;
+        bool walk = end > i + 1 && s->len > 0 && fused_permutation_runs();
+        std::vector<PermStep> steps;
+        for (uint64_t j = i; walk && j < end; ++j) {
+            const pp_channel& ch = channel_at(j);
+            const Prepared p = prepare(s, ch, adjoint);
+            if (!is_permutation_step(p)) {
+                walk = false;
+                break;
+            }
+            PermStep q{};
+            uint32_t gphase = 0;
+            for (uint32_t w = 0; w < s->W; ++w) {
+                q.gx[w] = ch.gx[w];
+                q.gz[w] = ch.gz[w];
+                q.supp |= ((ch.gx[w] | ch.gz[w]) != 0 ? 1u : 0u) << w;
+                gphase += (uint32_t)__builtin_popcountll(ch.gx[w] & ch.gz[w]);
+            }
+            q.gphase = gphase & 3u;
+            q.move = p.key_preserving ? 0u : 1u;
+            q.amp = p.key_preserving ? p.cos : p.sin;
+            steps.push_back(q);
+        }
+        if (walk) return permutation_run_device(s, steps.data(), (uint32_t)steps.size(), keep, len_after);
+        for (uint64_t j = i; j < end; ++j) {
+            const Prepared q = prepare(s, channel_at(j), adjoint);
+            if ((st = apply_one(s, channel_at(j), q, keep)) != PP_OK) return st;
+            if (len_after) len_after[j - i] = s->len;
+        }
+        return PP_OK;
     }
 
     // The layer pass after a run.
@@ -1312,11 +1458,7 @@ pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channel
     uint64_t i = 0;
     while (i < n_channels) {
         const uint64_t end = plan.start_run(s, i);
-        for (uint64_t j = i; j < end; ++j) {
-            const Prepared q = prepare(s, plan.channel_at(j), plan.adjoint);
-            if ((st = apply_one(s, plan.channel_at(j), q, keep)) != PP_OK) return st;
-            if (layer_len) layer_len[j] = s->len;
-        }
+        if ((st = plan.apply_run(s, i, end, keep, layer_len ? layer_len + i : nullptr)) != PP_OK) return st;
         if ((st = plan.finish_run(s)) != PP_OK) return st;
         if (layer_len && plan.finalizes) layer_len[end - 1] = s->len;
         i = end;

@@ -60,6 +60,52 @@ struct RescaleArgs {
     unsigned long long* stats;
 };
 
+// One step of a permutation run (a Clifford rotation or a key-preserving one) as the device walks it.
+struct PermStep {
+    uint64_t gx[2];
+    uint64_t gz[2];
+    double amp;       // move: the sine the moved rows take (partner_amplitude_on); else the rows' cosine factor
+    uint32_t supp;    // words of g's support
+    uint32_t gphase;  // |gx & gz| mod 4
+    uint32_t delta;   // device bucket delta of g
+    uint32_t move;    // 1: anticommuting rows move to k ^ g (the Clifford arm); 0: they are rescaled in place
+};
+constexpr uint32_t kMaxPermSteps = 256;  // steps per walk launch (a longer run takes several)
+constexpr uint32_t kDeadRow = 0xFFFFFFFFu;
+
+template <int W, typename T>
+struct PermWalkArgs {
+    uint64_t* x;  // walked in place
+    uint64_t* z;
+    T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    const uint32_t* len;
+    const PermStep* steps;
+    uint32_t num_steps;
+    KeepProgram keep;
+    uint32_t* dest;                 // per column position: the row's output bucket, or kDeadRow
+    uint32_t* counts;               // per output bucket: rows that land there
+    unsigned long long* deaths;     // per step: rows that step drops
+};
+
+template <int W, typename T>
+struct PermScatterArgs {
+    const uint64_t* x;
+    const uint64_t* z;
+    const T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    const uint32_t* len;
+    const uint32_t* dest;
+    uint64_t* ox;
+    uint64_t* oz;
+    T* oc;
+    uint64_t ocap;
+    const uint64_t* ostart;
+    uint32_t* olen;  // zero on entry: the write cursors, and the output lengths on exit
+};
+
 template <int W, typename T>
 struct RefineArgs {
     const uint64_t* x;
@@ -163,6 +209,10 @@ hipError_t launch_join_batch(hipStream_t st, const JoinArgs<W, T>* items, const 
                              unsigned long long* out);
 template <int W, typename T>
 hipError_t launch_rescale(hipStream_t st, const RescaleArgs<W, T>& a, uint32_t num_buckets);
+template <int W, typename T>
+hipError_t launch_perm_walk(hipStream_t st, const PermWalkArgs<W, T>& a, uint32_t num_buckets);
+template <int W, typename T>
+hipError_t launch_perm_scatter(hipStream_t st, const PermScatterArgs<W, T>& a, uint32_t num_buckets);
 template <int W, typename T>
 hipError_t launch_refine(hipStream_t st, const RefineArgs<W, T>& a);
 template <int W, typename T>

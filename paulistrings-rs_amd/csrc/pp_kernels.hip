@@ -15,7 +15,8 @@ namespace pp {
 // Bucket offsets of the next layout: a three-kernel exclusive scan over per-bucket sizes (tiles of
 // kScanTile buckets: tile sums, one workgroup scans the tile sums, then each tile scans itself).
 // mode 0: size = len[b]; mode 1: rotation output room len[b] + len[b ^ delta] (2 len[b] when delta = 0);
-// mode 2: coarsened bucket b < coarse_buckets gathers b + e * coarse_buckets for e < fold.
+// mode 2: coarsened bucket b < coarse_buckets gathers b + e * coarse_buckets for e < fold; mode 3: len[b] as a
+// resident segment (a permutation run's counts).
 // ---------------------------------------------------------------------------------------------------------------
 
 constexpr uint32_t kScanItems = 8;
@@ -36,12 +37,15 @@ struct OffsetsArgs {
 };
 
 // Mode 0 packs the buckets densely (the export layout); modes 1 (a join's outputs: at most both buckets of the
-// pair) and 2 (coarsening) allocate resident segments, rounded up to kSegAlign rows.
+// pair), 2 (coarsening) and 3 (a permutation run's destinations) allocate resident segments, rounded up to
+// kSegAlign rows.
 __device__ __forceinline__ uint64_t bucket_size(const OffsetsArgs& a, uint32_t b) {
     if (a.mode == 0) return a.len[b];
     uint64_t s = 0;
     if (a.mode == 1) {
         s = a.delta ? (uint64_t)a.len[b] + a.len[b ^ a.delta] : 2ull * a.len[b];
+    } else if (a.mode == 3) {
+        s = a.len[b];
     } else {
         for (uint32_t e = 0; e < a.fold; ++e) s += a.len[b + e * a.coarse_buckets];
     }
@@ -671,6 +675,92 @@ __global__ void __launch_bounds__(kBlock) k_rescale(const RescaleArgs<W, T> a) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// A permutation run (engine/bucketed/permutation_run.rs:1-4, :216-293, :297-...): steps that map distinct keys to
+// distinct keys, so no two rows ever meet and each row walks the whole run alone, in registers.  Per step a row
+// that anticommutes with g is either moved to k ^ g with c * a1(k) (a Clifford rotation, the join's arm that drops
+// the identity residue: the coefficient the join gives that key is 0 + c * a1(k), the same value) or rescaled to
+// c * cos (a key-preserving rotation, k_rescale's product); a row that commutes is unchanged.  A row is dropped
+// where the per-layer path drops it: after a step that changed it, when it no longer survives, and when it never
+// changed or first changes after step 0, where the first layer would have refused it unchanged.  The walk counts
+// every row's output bucket and every step's drops; k_perm_scatter then moves the rows into their buckets.
+// ---------------------------------------------------------------------------------------------------------------
+
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_perm_walk(const PermWalkArgs<W, T> a) {
+    __shared__ uint32_t died[kMaxPermSteps];
+    for (uint32_t j = threadIdx.x; j < a.num_steps; j += kBlock) died[j] = 0;
+    __syncthreads();
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = a.len[b];
+    const uint64_t s0 = a.start[b];
+    for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
+        const uint64_t i = s0 + r;
+        Key<W> k = load_key<W>(a.x, a.z, a.cap, i);
+        T c = a.c[i];
+        uint32_t dest = b;
+        bool tested = false;  // whether the row, as it now is, has been asked survives
+        int dead_at = -1;
+        for (uint32_t j = 0; j < a.num_steps; ++j) {
+            const PermStep& st = a.steps[j];
+            Key<W> g;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                g.x[w] = st.gx[w];
+                g.z[w] = st.gz[w];
+            }
+            if (!anticommutes_on<W>(k, g, st.supp)) continue;
+            if (!tested && j > 0 && !survives<W, T>(a.keep, k, c)) {
+                dead_at = 0;
+                break;
+            }
+            if (st.move) {
+                c = c * partner_amplitude_on<W, T>(k, g, st.supp, st.gphase, (T)st.amp);
+                k = key_xor<W>(k, g);
+                dest ^= st.delta;
+            } else {
+                c = c * (T)st.amp;
+            }
+            tested = true;
+            if (!survives<W, T>(a.keep, k, c)) {
+                dead_at = (int)j;
+                break;
+            }
+        }
+        if (dead_at < 0 && !tested && !survives<W, T>(a.keep, k, c)) dead_at = 0;
+        if (dead_at >= 0) {
+            atomicAdd(&died[dead_at], 1u);
+            a.dest[i] = kDeadRow;
+        } else {
+            store_key<W>(a.x, a.z, a.cap, i, k);
+            store_coeff<T>(a.c, i, c);
+            a.dest[i] = dest;
+            atomicAdd(&a.counts[dest], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < a.num_steps; j += kBlock)
+        if (died[j]) atomicAdd(&a.deaths[j], (unsigned long long)died[j]);
+}
+
+// Each surviving row to the next free slot of its output bucket (rows within a bucket are in no particular order).
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_perm_scatter(const PermScatterArgs<W, T> a) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = a.len[b];
+    const uint64_t s0 = a.start[b];
+    for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
+        const uint64_t i = s0 + r;
+        const uint32_t d = a.dest[i];
+        if (d == kDeadRow) continue;
+        const Key<W> k = load_key<W>(a.x, a.z, a.cap, i);
+        const T c = a.c[i];
+        const uint64_t pos = a.ostart[d] + atomicAdd(&a.olen[d], 1u);
+        store_key<W>(a.ox, a.oz, a.ocap, pos, k);
+        store_coeff<T>(a.oc, pos, c);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // Refine by one hash row: bucket b splits into b (row parity 0) and b + B (parity 1) in one pass; the parity-0
 // rows fill the segment from the front, the parity-1 rows from the back.
 // ---------------------------------------------------------------------------------------------------------------
@@ -906,6 +996,21 @@ hipError_t launch_rescale(hipStream_t st, const RescaleArgs<W, T>& a, uint32_t n
 }
 
 template <int W, typename T>
+hipError_t launch_perm_walk(hipStream_t st, const PermWalkArgs<W, T>& a, uint32_t num_buckets) {
+    if (num_buckets == 0 || a.num_steps == 0) return hipSuccess;
+    if (a.num_steps > kMaxPermSteps) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_perm_walk<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int W, typename T>
+hipError_t launch_perm_scatter(hipStream_t st, const PermScatterArgs<W, T>& a, uint32_t num_buckets) {
+    if (num_buckets == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_perm_scatter<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int W, typename T>
 hipError_t launch_refine(hipStream_t st, const RefineArgs<W, T>& a) {
     if (a.num_buckets == 0) return hipSuccess;
     hipLaunchKernelGGL((k_refine<W, T>), dim3(a.num_buckets), dim3(kBlock), 0, st, a);
@@ -938,6 +1043,8 @@ hipError_t launch_export(hipStream_t st, const ExportArgs<W, T>& a, uint32_t num
     template hipError_t launch_join_batch<W, T>(hipStream_t, const JoinArgs<W, T>*, const uint32_t*, uint32_t,     \
                                                 uint32_t, uint32_t, uint32_t, unsigned long long*);             \
     template hipError_t launch_rescale<W, T>(hipStream_t, const RescaleArgs<W, T>&, uint32_t);          \
+    template hipError_t launch_perm_walk<W, T>(hipStream_t, const PermWalkArgs<W, T>&, uint32_t);       \
+    template hipError_t launch_perm_scatter<W, T>(hipStream_t, const PermScatterArgs<W, T>&, uint32_t); \
     template hipError_t launch_refine<W, T>(hipStream_t, const RefineArgs<W, T>&);                      \
     template hipError_t launch_coarsen<W, T>(hipStream_t, const CoarsenArgs<W, T>&);                    \
     template hipError_t launch_expectation<W, T>(hipStream_t, const ExpectationArgs<W, T>&, uint32_t);  \

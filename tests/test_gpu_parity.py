@@ -139,6 +139,58 @@ def test_random_circuits_bitwise(num_qubits, dtype, direction):
     assert_same(o, p)
 
 
+def clifford_run_circuit(rng, num_qubits, steps):
+    """Long runs of one-to-one steps -- quarter turns (the Clifford arm) and half turns (key-preserving sign flips)
+    on one or two qubits -- between single generic rotations."""
+    c = Circuit(num_qubits)
+    for i in range(steps):
+        k = int(rng.integers(1, 3))
+        qs = [int(q) for q in rng.choice(num_qubits, size=k, replace=False)]
+        ps = "".join(rng.choice(list("XYZ"), size=k))
+        if i % 97 == 96:
+            theta = float(rng.uniform(-math.pi, math.pi))
+        else:
+            theta = float(rng.choice([math.pi / 2, -math.pi / 2, 3 * math.pi / 2, math.pi, -math.pi]))
+        c.pauli_rotation(ps, qs, theta)
+    return c
+
+
+@pytest.mark.parametrize("num_qubits", [40, 127])
+@pytest.mark.parametrize("dtype", [F64, F32])
+def test_permutation_runs_walked_in_one_pass(num_qubits, dtype):
+    """Runs of up to ~100 one-to-one steps, walked in one pass per run (and a run longer than one walk launch's
+    256 steps), against the oracle stepping layer by layer: keys, coefficients and every layer's length."""
+    announce(f"permutation runs n={num_qubits} {np.dtype(dtype).name}")
+    rng = np.random.default_rng(71 + num_qubits)
+    x, z, c = random_sum(rng, 4000, num_qubits, 0.1, dtype)
+    circ = clifford_run_circuit(rng, num_qubits, 300)
+    o, p = run_both(num_qubits, x, z, c, circ, "heisenberg", eps=1e-3, max_weight=num_qubits // 4)
+    assert_same(o, p)
+    long_run = Circuit(num_qubits)
+    for i in range(600):
+        q = int(rng.integers(0, num_qubits - 1))
+        long_run.pauli_rotation("ZZ" if i % 2 else "XY", [q, q + 1], math.pi / 2 if i % 3 else -math.pi / 2)
+    o, p = run_both(num_qubits, x, z, c, long_run, "forward", eps=1e-3, max_weight=num_qubits // 3)
+    assert_same(o, p)
+
+
+def test_permutation_runs_match_the_layer_by_layer_device_path(monkeypatch):
+    """The one-pass walk and the per-layer join (PP_PERMUTATION_RUNS=layers) give the same sum bit for bit, and
+    both the oracle's, on kicked-Ising steps whose 144 quarter-turn RZZ layers form one run each (peak 1.6M terms)."""
+    announce("permutation runs: walk vs layers")
+    rng = np.random.default_rng(5)
+    x, z, c = random_sum(rng, 500, 127, 0.02, F64)
+    circ = circuits.heavy_hex_kicked_ising(4, 0.8)
+    o, walk = run_both(127, x, z, c, circ, "heisenberg", eps=3e-3, max_weight=12)
+    assert_same(o, walk)
+    monkeypatch.setenv("PP_PERMUTATION_RUNS", "layers")
+    host = PauliSum(127, x, z, c)
+    stats = host.propagate_with_stats(circ, product_policy(3e-3, 12), "heisenberg")
+    assert np.array_equal(host.x_array(), walk[0]) and np.array_equal(host.z_array(), walk[1])
+    assert np.array_equal(host.coefficients_array(), walk[2])
+    assert stats.terms_out == walk[3].terms_out
+
+
 @pytest.mark.parametrize("dtype", [F64, F32])
 def test_untruncated_growth_bitwise(dtype):
     announce(f"untruncated growth {np.dtype(dtype).name}")

@@ -686,7 +686,8 @@ __global__ void __launch_bounds__(kBlock) k_rescale(const RescaleArgs<W, T> a) {
 // ---------------------------------------------------------------------------------------------------------------
 
 template <int W, typename T>
-__global__ void __launch_bounds__(kBlock) k_perm_walk(const PermWalkArgs<W, T> a) {
+__global__ void __launch_bounds__(kBlock) k_perm_walk(const PermWalkArgs<W, T> a,
+                                                      const PermStep* __restrict__ steps) {
     __shared__ uint32_t died[kMaxPermSteps];
     for (uint32_t j = threadIdx.x; j < a.num_steps; j += kBlock) died[j] = 0;
     __syncthreads();
@@ -700,18 +701,20 @@ __global__ void __launch_bounds__(kBlock) k_perm_walk(const PermWalkArgs<W, T> a
         uint32_t dest = b;
         bool tested = false;  // whether the row, as it now is, has been asked survives
         int dead_at = -1;
+        // Every lane runs every step (a dropped row idles): the step index stays uniform, so the steps are read
+        // through the scalar cache rather than by a vector load per lane and step.
         for (uint32_t j = 0; j < a.num_steps; ++j) {
-            const PermStep& st = a.steps[j];
+            const PermStep& st = steps[j];
             Key<W> g;
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 g.x[w] = st.gx[w];
                 g.z[w] = st.gz[w];
             }
-            if (!anticommutes_on<W>(k, g, st.supp)) continue;
+            if (dead_at >= 0 || !anticommutes_on<W>(k, g, st.supp)) continue;
             if (!tested && j > 0 && !survives<W, T>(a.keep, k, c)) {
                 dead_at = 0;
-                break;
+                continue;
             }
             if (st.move) {
                 c = c * partner_amplitude_on<W, T>(k, g, st.supp, st.gphase, (T)st.amp);
@@ -721,10 +724,7 @@ __global__ void __launch_bounds__(kBlock) k_perm_walk(const PermWalkArgs<W, T> a
                 c = c * (T)st.amp;
             }
             tested = true;
-            if (!survives<W, T>(a.keep, k, c)) {
-                dead_at = (int)j;
-                break;
-            }
+            if (!survives<W, T>(a.keep, k, c)) dead_at = (int)j;
         }
         if (dead_at < 0 && !tested && !survives<W, T>(a.keep, k, c)) dead_at = 0;
         if (dead_at >= 0) {
@@ -999,7 +999,7 @@ template <int W, typename T>
 hipError_t launch_perm_walk(hipStream_t st, const PermWalkArgs<W, T>& a, uint32_t num_buckets) {
     if (num_buckets == 0 || a.num_steps == 0) return hipSuccess;
     if (a.num_steps > kMaxPermSteps) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_perm_walk<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((k_perm_walk<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a, a.steps);
     return hipGetLastError();
 }
 

@@ -173,45 +173,55 @@ __global__ void __launch_bounds__(kBlock) k_topn_gather_ties(const TopnTieArgs<W
     }
 }
 
+// Whether the term at column position i (coefficient c) survives the selection: above the boundary bin on the
+// decided bits, or in it and flagged (without flags, every term in it).
+template <int W, typename T>
+__device__ __forceinline__ bool retain_keeps(const TopnRetainArgs<W, T>& a, const TopnState& st, T c, uint64_t i) {
+    const uint64_t u = a.score ? score_bits(a.score[i]) : magnitude_bits<T>(c);
+    const bool nan = !a.score && is_nan_bits(u);
+    const uint64_t um = u & st.mask;
+    if (st.keep_all_finite) return !nan;
+    if (nan) return false;
+    if (um > st.prefix) return true;
+    if (um == st.prefix) return a.tie_keep == nullptr || a.tie_keep[i] != 0;
+    return false;
+}
+
+// In-place retain, 2 kBlock rows per step: both rows' loads are issued before the step's one scan, and a dropped
+// row's key is never read.  Every write lands at or before its row's own position, and every row of the step was
+// read before the scan's barrier.
 template <int W, typename T>
 __global__ void __launch_bounds__(kBlock) k_topn_retain(const TopnRetainArgs<W, T> a) {
-    __shared__ uint32_t scan_scratch[kWaves];
+    __shared__ uint32_t scan_scratch[2 * kWaves];
     const uint32_t b = blockIdx.x;
     const uint32_t n = a.len[b];
     const uint64_t s = a.start[b];
     const TopnState st = *a.state;
     uint32_t cur = 0;
-    for (uint32_t base = 0; base < n; base += kBlock) {
-        const uint32_t r = base + threadIdx.x;
-        Key<W> k;
-        T c = T(0);
-        bool keep = false;
-        if (r < n) {
-            k = load_key<W>(a.x, a.z, a.cap, s + r);
-            c = a.c[s + r];
-            const uint64_t u = rank_bits<T>(a.c, a.score, s + r);
-            const bool nan = !a.score && is_nan_bits(u);
-            // above the boundary bin on the decided bits: kept; in it: kept when flagged (or, without flags,
-            // every one of them)
-            const uint64_t um = u & st.mask;
-            if (st.keep_all_finite) keep = !nan;
-            else if (nan) keep = false;
-            else if (um > st.prefix) keep = true;
-            else if (um == st.prefix) keep = a.tie_keep == nullptr || a.tie_keep[s + r] != 0;
-        }
-        bool f[1] = {keep};
-        uint32_t off[1], tot[1];
-        block_flag_scan<1>(f, off, tot, scan_scratch);  // every read of the chunk precedes its writes
-        if (keep && cur + off[0] != r) {
+    for (uint32_t base = 0; base < n; base += 2 * kBlock) {
+        const uint32_t r0 = base + threadIdx.x, r1 = r0 + kBlock;
+        T c0 = T(0), c1 = T(0);
+        if (r0 < n) c0 = a.c[s + r0];
+        if (r1 < n) c1 = a.c[s + r1];
+        const bool keep0 = r0 < n && retain_keeps<W, T>(a, st, c0, s + r0);
+        const bool keep1 = r1 < n && retain_keeps<W, T>(a, st, c1, s + r1);
+        Key<W> k0, k1;
+        if (keep0) k0 = load_key<W>(a.x, a.z, a.cap, s + r0);
+        if (keep1) k1 = load_key<W>(a.x, a.z, a.cap, s + r1);
+        bool f[2] = {keep0, keep1};
+        uint32_t off[2], tot[2];
+        block_flag_scan<2>(f, off, tot, scan_scratch);
+        if (keep0 && cur + off[0] != r0) {
             const uint64_t pos = s + cur + off[0];
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                a.x[w * a.cap + pos] = k.x[w];
-                a.z[w * a.cap + pos] = k.z[w];
-            }
-            a.c[pos] = c;
+            store_key<W>(a.x, a.z, a.cap, pos, k0);
+            store_coeff<T>(a.c, pos, c0);
         }
-        cur += tot[0];
+        if (keep1 && cur + tot[0] + off[1] != r1) {
+            const uint64_t pos = s + cur + tot[0] + off[1];
+            store_key<W>(a.x, a.z, a.cap, pos, k1);
+            store_coeff<T>(a.c, pos, c1);
+        }
+        cur += tot[0] + tot[1];
     }
     if (threadIdx.x == 0) a.len[b] = cur;
 }

@@ -24,6 +24,7 @@ struct TopnState {
 constexpr int kTopnDigitBits = 11;
 constexpr int kTopnBins = 1 << kTopnDigitBits;
 constexpr int kTopnPasses = 6;  // digits at shifts 53, 42, 31, 20, 9 (11 bits) and 0 (9 bits)
+constexpr uint32_t kHistBucketsMax = 256;  // buckets per histogram workgroup (one length per thread)
 
 __host__ __device__ inline int topn_shift(int pass) { return pass < 5 ? 53 - 11 * pass : 0; }
 __host__ __device__ inline int topn_bits(int pass) { return pass < 5 ? 11 : 9; }

@@ -114,26 +114,65 @@ __device__ __forceinline__ void select_digit(TopnState* s, unsigned long long* h
 }
 
 // One radix pass: each workgroup histograms the digit of its buckets' terms that match the prefix found so far and
-// adds its counts to the global histogram; k_topn_select picks the digit.
+// adds its counts to the global histogram; k_topn_select picks the digit.  The workgroup walks its buckets' rows
+// as one list, so that its threads' loads do not wait bucket by bucket.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
     __shared__ uint32_t h[kTopnBins];
+    __shared__ uint32_t first[kHistBucketsMax + 1];  // the workgroup's rows before each of its buckets
+    __shared__ uint64_t seg[kHistBucketsMax];        // each bucket's segment start
+    __shared__ uint32_t wave_rows[kWaves];
     for (int i = threadIdx.x; i < kTopnBins; i += kBlock) h[i] = 0;
-    __syncthreads();
     const TopnState st = *a.state;
+    const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
+    const uint32_t nb = min(a.num_buckets, b_lo + a.buckets_per_block) - b_lo;  // <= kHistBucketsMax
+    // The buckets' rows as one list: an inclusive scan of their lengths (one per thread).
+    const uint32_t t = threadIdx.x;
+    uint32_t v = t < nb ? a.len[b_lo + t] : 0u;
+    if (t < nb) seg[t] = a.start[b_lo + t];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, 64);
+        if (lane_id() >= (uint32_t)d) v += o;
+    }
+    if (lane_id() == 63) wave_rows[wave_id()] = v;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        before += (uint32_t)w < wave_id() ? wave_rows[w] : 0u;
+        total += wave_rows[w];
+    }
+    if (t < nb) first[t + 1] = before + v;
+    if (t == 0) first[0] = 0;
+    __syncthreads();
     if (!st.keep_all_finite) {
         const int shift = topn_shift(a.pass);
         const uint64_t dmask = (1ull << topn_bits(a.pass)) - 1ull;
-        const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
-        const uint32_t b_hi = min(a.num_buckets, b_lo + a.buckets_per_block);
-        for (uint32_t b = b_lo; b < b_hi; ++b) {
-            const uint32_t n = a.len[b];
-            const uint64_t s = a.start[b];
-            for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
-                const uint64_t u = rank_bits<T>(a.c, a.score, s + r);
-                const bool match = (a.score || !is_nan_bits(u)) && (u & st.mask) == st.prefix;
-                const uint32_t bin = (uint32_t)((u >> shift) & dmask);
-                if (match) atomicAdd(&h[bin], 1u);
+        // Row g of the list is row g - first[j] of bucket j; a thread's rows only move forward, so j does too.
+        // Four rows per thread per step, their loads issued together.
+        uint32_t j = 0;
+        for (uint32_t g0 = t; g0 < total; g0 += 4 * kBlock) {
+            uint64_t pos[4];
+            bool in[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t g = g0 + (uint32_t)q * kBlock;
+                in[q] = g < total;
+                if (in[q]) {
+                    while (first[j + 1] <= g) ++j;
+                    pos[q] = seg[j] + (g - first[j]);
+                }
+            }
+            uint64_t u[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (in[q]) u[q] = rank_bits<T>(a.c, a.score, pos[q]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (!in[q]) continue;
+                const bool match = (a.score || !is_nan_bits(u[q])) && (u[q] & st.mask) == st.prefix;
+                if (match) atomicAdd(&h[(uint32_t)((u[q] >> shift) & dmask)], 1u);
             }
         }
     }

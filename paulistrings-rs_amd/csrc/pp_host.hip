@@ -300,6 +300,12 @@ struct pp_sum {
     uint64_t len = 0;
     uint32_t max_bucket = 0;
     bool stale = false;  // len and max_bucket await the last layer's counters (pp_apply_rotation_to)
+    // A TopN selection whose retain is left to the next join (inside pp_propagate only): len is then the kept
+    // count, phys_len the rows the columns still hold, and max_bucket their longest bucket.
+    bool pending_topn = false;
+    uint64_t phys_len = 0;
+    const uint8_t* pending_flags = nullptr;
+    std::vector<uint64_t> tie_kept;  // TopN's kept boundary positions, the source of an asynchronous copy
     uint64_t* d_tiles = nullptr;  // scratch of the bucket-offset scan
     uint64_t tiles_cap = 0;
     unsigned long long* d_stats = nullptr;
@@ -559,6 +565,8 @@ pp_status coarsen_to(pp_sum* s, int bits) {
     return PP_OK;
 }
 
+pp_status materialize_topn(pp_sum* s);  // pp_topn_impl.inc
+
 // The device's own bucket count: about the context's bucket_target rows per bucket, with a factor-8 hysteresis, and
 // fine enough that the largest pair fits one workgroup's LDS.
 pp_status settle_device_bits(pp_sum* s) {
@@ -566,10 +574,15 @@ pp_status settle_device_bits(pp_sum* s) {
     int want = 0;
     while (want < s->max_bits && s->len > target * (1ull << want)) ++want;
     pp_status st;
+    // refine and coarsen copy every row they hold: a pending selection is applied first
+    const bool coarsen = s->len * 8 < target * (1ull << s->device_bits) && want < s->device_bits;
+    if ((want > s->device_bits || coarsen || 2ull * s->max_bucket > kMaxPairRows) &&
+        (st = materialize_topn(s)) != PP_OK)
+        return st;
     if (want > s->device_bits) {
         while (s->device_bits < want)
             if ((st = refine_once(s)) != PP_OK) return st;
-    } else if (s->len * 8 < target * (1ull << s->device_bits) && want < s->device_bits) {
+    } else if (coarsen) {
         if ((st = coarsen_to(s, want)) != PP_OK) return st;
     }
     while (2ull * s->max_bucket > kMaxPairRows) {
@@ -673,7 +686,8 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
     pp_status st;
     const int in = src->cur;
     const uint32_t B = 1u << src->device_bits;
-    if ((st = ensure_columns(dst, out, 2 * src->len + (uint64_t)kSegAlign * B)) != PP_OK) return st;
+    const uint64_t rows_held = src->pending_topn ? src->phys_len : src->len;
+    if ((st = ensure_columns(dst, out, 2 * rows_held + (uint64_t)kSegAlign * B)) != PP_OK) return st;
     if ((st = ensure_buckets(dst, out, B)) != PP_OK) return st;
     const uint32_t delta = bucket_of_host(src, ch.gx, ch.gz, src->device_bits);
     // the scan also clears dst's counters
@@ -747,6 +761,8 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         a.redo = dst->d_redo;
         a.redo_count = dst->d_stats + 4;
         a.count_totals = count_in_join ? 1u : 0u;
+        a.filter = src->pending_topn ? src->d_topn : nullptr;
+        a.filter_flags = src->pending_topn ? src->pending_flags : nullptr;
         hipError_t le = launch_rotation_join<W, T>(ctx->stream, a, ctx->tuning.groups_per_cu, ctx->tuning.join_block);
         if (le != hipSuccess || scap >= lcap) return le;  // a first launch staging lcap never queues a pair
         a.redo_mode = 1;
@@ -768,6 +784,7 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
 pp_status device_layer(pp_sum* s, const pp_channel& ch, const Prepared& p, const KeepProgram& keep) {
     pp_context* ctx = s->ctx;
     pp_status st;
+    if (p.key_preserving && (st = materialize_topn(s)) != PP_OK) return st;
     if ((st = settle_device_bits(s)) != PP_OK) return st;
     const uint32_t B = 1u << s->device_bits;
     if (p.key_preserving) {
@@ -796,6 +813,7 @@ pp_status device_layer(pp_sum* s, const pp_channel& ch, const Prepared& p, const
     }
     const int out = 1 - s->cur;
     if ((st = launch_join(s, s, out, ch, p, keep)) != PP_OK) return st;
+    s->pending_topn = false;  // the join read its input through the selection
     if ((st = read_stats(s)) != PP_OK) return st;
     if (s->h_stats[2]) return fail(ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
     s->len = s->h_stats[0];
@@ -819,7 +837,7 @@ pp_status permutation_run_device(pp_sum* s, const PermStep* steps, uint32_t n_st
                                  uint64_t* len_after) {
     pp_context* ctx = s->ctx;
     pp_status st;
-    if ((st = settle_device_bits(s)) != PP_OK) return st;
+    if ((st = materialize_topn(s)) != PP_OK || (st = settle_device_bits(s)) != PP_OK) return st;
     const uint32_t B = 1u << s->device_bits;
     uint32_t *d_dest = nullptr, *d_counts = nullptr;
     unsigned long long* d_deaths = nullptr;
@@ -901,10 +919,12 @@ pp_status permutation_run_device(pp_sum* s, const PermStep* steps, uint32_t n_st
     return PP_OK;
 }
 
-// Bring len and max_bucket up to date after an asynchronous out-of-place layer.
+// Bring len and max_bucket up to date after an asynchronous out-of-place layer (and apply a selection an
+// interrupted propagation left pending).
 pp_status refresh(pp_sum* s) {
-    if (!s->stale) return PP_OK;
     pp_status st;
+    if (s->pending_topn && (st = materialize_topn(s)) != PP_OK) return st;
+    if (!s->stale) return PP_OK;
     if ((st = read_stats(s)) != PP_OK) return st;
     s->stale = false;
     if (s->h_stats[2]) return fail(s->ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
@@ -917,7 +937,8 @@ pp_status apply_one(pp_sum* s, const pp_channel& ch, const Prepared& p, const Ke
     return device_layer(s, ch, p, keep);
 }
 
-pp_status topn_device(pp_sum* s, uint64_t n, const float* score = nullptr);  // pp_topn_impl.inc
+pp_status topn_device(pp_sum* s, uint64_t n, const float* score = nullptr,
+                      bool defer = false);  // pp_topn_impl.inc
 pp_status approx_topn_device(pp_sum* s, uint64_t n);  // pp_topn_impl.inc
 
 pp_status check_channels(pp_sum* s, const pp_channel* channels, uint64_t n_channels) {
@@ -1021,8 +1042,9 @@ struct RunPlanner {
         return PP_OK;
     }
 
-    // The layer pass after a run.
-    pp_status finish_run(pp_sum* s) const;
+    // The layer pass after a run.  With defer, an exact TopN leaves its retain to the next join (the caller applies
+    // it with materialize_topn before anything else reads the rows).
+    pp_status finish_run(pp_sum* s, bool defer = false) const;
 };
 
 }  // namespace
@@ -1459,7 +1481,7 @@ pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channel
     while (i < n_channels) {
         const uint64_t end = plan.start_run(s, i);
         if ((st = plan.apply_run(s, i, end, keep, layer_len ? layer_len + i : nullptr)) != PP_OK) return st;
-        if ((st = plan.finish_run(s)) != PP_OK) return st;
+        if ((st = plan.finish_run(s, end < n_channels)) != PP_OK) return st;
         if (layer_len && plan.finalizes) layer_len[end - 1] = s->len;
         i = end;
     }
@@ -1484,9 +1506,12 @@ pp_status pp_truncate_approx_topn(pp_sum* s, uint64_t n) {
     return approx_topn_device(s, n);
 }
 
-pp_status RunPlanner::finish_run(pp_sum* s) const {
+pp_status RunPlanner::finish_run(pp_sum* s, bool defer) const {
     if (!finalizes) return PP_OK;
-    return policy->has_topn == 2 ? pp_truncate_approx_topn(s, policy->topn) : pp_truncate_topn(s, policy->topn);
+    if (policy->has_topn == 2) return pp_truncate_approx_topn(s, policy->topn);
+    if (!defer) return pp_truncate_topn(s, policy->topn);
+    if (s->len <= policy->topn) return PP_OK;
+    return topn_device(s, policy->topn, nullptr, true);
 }
 
 pp_status pp_expectation_product(pp_sum* s, const uint64_t* axis_x, const uint64_t* axis_z, const uint64_t* negative,

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "pp_device.h"
+#include "pp_topn.h"
 
 namespace pp {
 
@@ -43,6 +44,10 @@ struct JoinArgs {
     uint32_t redo_mode;              // 0: every pair; 1: the queued pairs, staging up to scap = the largest pair
     uint32_t count_totals;           // each pair adds its rows to stats[0] and maxes stats[1] (few buckets: no
                                      // k_reduce_lens launch)
+    // A finished TopN selection not yet applied to the input (null: none): rows it drops are not read further, as
+    // though its retain had run (|c| ranking only).
+    const TopnState* filter;
+    const uint8_t* filter_flags;  // its boundary-bin flags per input position, or null
 };
 
 template <int W, typename T>

@@ -412,6 +412,8 @@ __device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, 
     }
     const uint32_t supp = support_words<W>(g);
     const uint32_t fold_g = key_fold<W>(g);
+    TopnState fs{};
+    if (a.filter) fs = *a.filter;
 
     uint32_t p = pair_at(t);
     PairMeta m = read_meta(a, p, load_meta(a, p));
@@ -449,8 +451,12 @@ __device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, 
         // ---- phase A over the current chunk ----
         if (!overflow) {  // uniform
             const uint32_t r = base + threadIdx.x;
-            const bool valid = r < n;
+            bool valid = r < n;
             const uint32_t j = r >= m.n0 ? 1u : 0u;
+            if (valid && a.filter) {  // uniform test: a pending TopN retain
+                const uint64_t u = magnitude_bits<T>(cb.c);
+                valid = topn_keeps(fs, u, is_nan_bits(u), a.filter_flags, j ? m.s1 + (r - m.n0) : m.s0 + r);
+            }
             bool anti = false, keep = false;
             if (valid) {
                 anti = anticommutes_on<W>(cb.k, g, supp);

@@ -21,6 +21,27 @@ struct TopnState {
     unsigned long long gathered;  // tie terms written by k_topn_gather_ties
 };
 
+__device__ __forceinline__ bool is_nan_bits(uint64_t u) { return (u & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull; }
+
+template <typename T>
+__device__ __forceinline__ uint64_t magnitude_bits(T c) {
+    const double v = (double)c;  // norm_sqr widens to f64 first (coefficient.rs:121-125)
+    return (uint64_t)__double_as_longlong(v * v);
+}
+
+// Whether a term whose ranking key is u (at column position i) survives a finished selection: above the boundary
+// bin on the decided bits, or in it and flagged (without flags, every term in it); with at most n finite terms every
+// finite one; a NaN |c|^2 never (a score key is never NaN-coded: score_nan marks the |c| case).
+__device__ __forceinline__ bool topn_keeps(const TopnState& st, uint64_t u, bool score_nan,
+                                           const uint8_t* tie_keep, uint64_t i) {
+    if (st.keep_all_finite) return !score_nan;
+    if (score_nan) return false;
+    const uint64_t um = u & st.mask;
+    if (um > st.prefix) return true;
+    if (um == st.prefix) return tie_keep == nullptr || tie_keep[i] != 0;
+    return false;
+}
+
 constexpr int kTopnDigitBits = 11;
 constexpr int kTopnBins = 1 << kTopnDigitBits;
 constexpr int kTopnPasses = 6;  // digits at shifts 53, 42, 31, 20, 9 (11 bits) and 0 (9 bits)

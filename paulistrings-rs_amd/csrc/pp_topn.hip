@@ -1,5 +1,6 @@
-// pp_topn.hip -- TopN kernels: a six-pass radix select on the IEEE bits of |c|^2 (non-negative doubles order as
-// their bits), the gather of the terms tied at the boundary, and the in-place retain.
+// pp_topn.hip -- TopN kernels: a radix select on the IEEE bits of |c|^2 (non-negative doubles order as their bits;
+// three 11-bit passes, six for a crowded boundary bin), the gather of the boundary bin's terms, and the in-place
+// retain.
 //
 // Reference semantics (truncation/builtin.rs:73-129, :152-239): with t the n-th largest |c|^2, keep every term above
 // t and the first n - above of the terms at t in canonical order (the reference's bucket ascending, then the
@@ -7,14 +8,6 @@
 #include "pp_topn.h"
 
 namespace pp {
-
-__device__ __forceinline__ bool is_nan_bits(uint64_t u) { return (u & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull; }
-
-template <typename T>
-__device__ __forceinline__ uint64_t magnitude_bits(T c) {
-    const double v = (double)c;  // norm_sqr widens to f64 first (coefficient.rs:121-125)
-    return (uint64_t)__double_as_longlong(v * v);
-}
 
 // A float score as a key in the high half of a u64 that orders as the score (a NaN score sorts lowest): the learned
 // TopN pass ranks terms by their score through the same selection as TopN.
@@ -217,13 +210,7 @@ __global__ void __launch_bounds__(kBlock) k_topn_gather_ties(const TopnTieArgs<W
 template <int W, typename T>
 __device__ __forceinline__ bool retain_keeps(const TopnRetainArgs<W, T>& a, const TopnState& st, T c, uint64_t i) {
     const uint64_t u = a.score ? score_bits(a.score[i]) : magnitude_bits<T>(c);
-    const bool nan = !a.score && is_nan_bits(u);
-    const uint64_t um = u & st.mask;
-    if (st.keep_all_finite) return !nan;
-    if (nan) return false;
-    if (um > st.prefix) return true;
-    if (um == st.prefix) return a.tie_keep == nullptr || a.tie_keep[i] != 0;
-    return false;
+    return topn_keeps(st, u, !a.score && is_nan_bits(u), a.tie_keep, i);
 }
 
 // In-place retain, 2 kBlock rows per step: both rows' loads are issued before the step's one scan, and a dropped

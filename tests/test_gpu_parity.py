@@ -291,6 +291,21 @@ def test_topn_inside_propagation():
     assert_same(o, p)
 
 
+@pytest.mark.parametrize("num_qubits,dtype", [(40, F64), (127, F64), (127, F32)])
+def test_topn_every_run_with_every_layer_kind(num_qubits, dtype):
+    """TopN after every run, where the next run is a join layer (which reads its input through the selection), a
+    key-preserving layer or a permutation run (both apply the selection first), or a refinement of the device
+    buckets: keys, coefficients and every layer's length against the oracle."""
+    announce(f"topn every run n={num_qubits} {np.dtype(dtype).name}")
+    rng = np.random.default_rng(83 + num_qubits)
+    x, z, c = random_sum(rng, 3000, num_qubits, 0.1, dtype)
+    circ = random_circuit(rng, num_qubits, 60)
+    o, p = run_both(num_qubits, x, z, c, circ, "heisenberg", eps=1e-6, max_weight=num_qubits // 3, topn=2500)
+    assert_same(o, p)
+    o, p = run_both(num_qubits, x, z, c, circ, "forward", eps=1e-6, topn=20000)
+    assert_same(o, p)
+
+
 @pytest.mark.parametrize("case", ["normal_w2_f64", "ties_w1_f64", "normal_w2_f32", "big_w2_f64"])
 def test_approx_topn_standalone(case):
     announce(f"approx topn {case}")

@@ -81,6 +81,8 @@ struct TopnTieArgs {
     uint64_t* tie_pos;
     uint64_t* tie_rank;  // the rank key of each gathered term (the boundary bin's terms need not be equal)
     uint64_t tie_cap;
+    uint32_t num_buckets;
+    uint32_t buckets_per_block;  // at most kHistBucketsMax
 };
 
 template <int W, typename T>

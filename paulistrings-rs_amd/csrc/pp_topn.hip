@@ -109,36 +109,63 @@ __device__ __forceinline__ void select_digit(TopnState* s, unsigned long long* h
 // One radix pass: each workgroup histograms the digit of its buckets' terms that match the prefix found so far and
 // adds its counts to the global histogram; k_topn_select picks the digit.  The workgroup walks its buckets' rows
 // as one list, so that its threads' loads do not wait bucket by bucket.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
-    __shared__ uint32_t h[kTopnBins];
-    __shared__ uint32_t first[kHistBucketsMax + 1];  // the workgroup's rows before each of its buckets
-    __shared__ uint64_t seg[kHistBucketsMax];        // each bucket's segment start
-    __shared__ uint32_t wave_rows[kWaves];
-    for (int i = threadIdx.x; i < kTopnBins; i += kBlock) h[i] = 0;
-    const TopnState st = *a.state;
-    const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
-    const uint32_t nb = min(a.num_buckets, b_lo + a.buckets_per_block) - b_lo;  // <= kHistBucketsMax
-    // The buckets' rows as one list: an inclusive scan of their lengths (one per thread).
+// The rows of buckets [b_lo, b_lo + nb) (nb <= kHistBucketsMax) as one list: first[j] rows precede bucket j, whose
+// segment starts at seg[j]; returns the rows in all.  Every thread of the workgroup calls it (two barriers).
+struct RowList {
+    uint32_t first[kHistBucketsMax + 1];
+    uint64_t seg[kHistBucketsMax];
+    uint32_t wave_rows[kWaves];
+};
+
+__device__ __forceinline__ uint32_t list_rows(const uint64_t* start, const uint32_t* len, uint32_t b_lo, uint32_t nb,
+                                              RowList& l) {
     const uint32_t t = threadIdx.x;
-    uint32_t v = t < nb ? a.len[b_lo + t] : 0u;
-    if (t < nb) seg[t] = a.start[b_lo + t];
+    uint32_t v = t < nb ? len[b_lo + t] : 0u;
+    if (t < nb) l.seg[t] = start[b_lo + t];
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const uint32_t o = __shfl_up(v, d, 64);
         if (lane_id() >= (uint32_t)d) v += o;
     }
-    if (lane_id() == 63) wave_rows[wave_id()] = v;
+    if (lane_id() == 63) l.wave_rows[wave_id()] = v;
     __syncthreads();
     uint32_t before = 0, total = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
-        before += (uint32_t)w < wave_id() ? wave_rows[w] : 0u;
-        total += wave_rows[w];
+        before += (uint32_t)w < wave_id() ? l.wave_rows[w] : 0u;
+        total += l.wave_rows[w];
     }
-    if (t < nb) first[t + 1] = before + v;
-    if (t == 0) first[0] = 0;
+    if (t < nb) l.first[t + 1] = before + v;
+    if (t == 0) l.first[0] = 0;
     __syncthreads();
+    return total;
+}
+
+// Column positions of list rows g0 + q * kBlock (q < 4) that exist; j is the caller's bucket cursor (rows only move
+// forward, so it does too).
+__device__ __forceinline__ void list_positions(const RowList& l, uint32_t total, uint32_t g0, uint32_t& j,
+                                               uint64_t (&pos)[4], bool (&in)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t g = g0 + (uint32_t)q * kBlock;
+        in[q] = g < total;
+        if (in[q]) {
+            while (l.first[j + 1] <= g) ++j;
+            pos[q] = l.seg[j] + (g - l.first[j]);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
+    __shared__ uint32_t h[kTopnBins];
+    __shared__ RowList list;
+    for (int i = threadIdx.x; i < kTopnBins; i += kBlock) h[i] = 0;
+    const TopnState st = *a.state;
+    const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
+    const uint32_t nb = min(a.num_buckets, b_lo + a.buckets_per_block) - b_lo;  // <= kHistBucketsMax
+    const uint32_t t = threadIdx.x;
+    const uint32_t total = list_rows(a.start, a.len, b_lo, nb, list);
     if (!st.keep_all_finite) {
         const int shift = topn_shift(a.pass);
         const uint64_t dmask = (1ull << topn_bits(a.pass)) - 1ull;
@@ -148,15 +175,7 @@ __global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
         for (uint32_t g0 = t; g0 < total; g0 += 4 * kBlock) {
             uint64_t pos[4];
             bool in[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t g = g0 + (uint32_t)q * kBlock;
-                in[q] = g < total;
-                if (in[q]) {
-                    while (first[j + 1] <= g) ++j;
-                    pos[q] = seg[j] + (g - first[j]);
-                }
-            }
+            list_positions(list, total, g0, j, pos, in);
             uint64_t u[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q)
@@ -180,28 +199,39 @@ __global__ void __launch_bounds__(kBlock) k_topn_select(TopnState* s, unsigned l
     select_digit(s, hist, pass, n);
 }
 
+// The terms of the boundary bin: rank key equal to the prefix on the decided bits (a NaN never matches a finite
+// prefix: its exponent bits are all ones).  Buckets are walked as in k_topn_hist.
 template <int W, typename T>
 __global__ void __launch_bounds__(kBlock) k_topn_gather_ties(const TopnTieArgs<W, T> a) {
-    const uint32_t b = blockIdx.x;
-    const uint32_t n = a.len[b];
-    const uint64_t s = a.start[b];
-    // the terms of the boundary bin: rank key equal to the prefix on the decided bits (a NaN never matches a
-    // finite prefix: its exponent bits are all ones)
+    __shared__ RowList list;
     const uint64_t t = a.state->prefix, mask = a.state->mask;
-    for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
-        const uint64_t u = rank_bits<T>(a.c, a.score, s + r);
-        if ((u & mask) != t || (!a.score && is_nan_bits(u))) continue;
-        const uint64_t slot = atomicAdd(&a.state->gathered, 1ull);
-        if (slot >= a.tie_cap) continue;
-        const Key<W> k = load_key<W>(a.x, a.z, a.cap, s + r);
+    const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
+    const uint32_t nb = min(a.num_buckets, b_lo + a.buckets_per_block) - b_lo;  // <= kHistBucketsMax
+    const uint32_t total = list_rows(a.start, a.len, b_lo, nb, list);
+    uint32_t j = 0;
+    for (uint32_t g0 = threadIdx.x; g0 < total; g0 += 4 * kBlock) {
+        uint64_t pos[4];
+        bool in[4];
+        list_positions(list, total, g0, j, pos, in);
+        uint64_t u[4];
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            a.tie_x[slot * W + w] = k.x[w];
-            a.tie_z[slot * W + w] = k.z[w];
+        for (int q = 0; q < 4; ++q)
+            if (in[q]) u[q] = rank_bits<T>(a.c, a.score, pos[q]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!in[q] || (u[q] & mask) != t || (!a.score && is_nan_bits(u[q]))) continue;
+            const uint64_t slot = atomicAdd(&a.state->gathered, 1ull);
+            if (slot >= a.tie_cap) continue;
+            const Key<W> k = load_key<W>(a.x, a.z, a.cap, pos[q]);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                a.tie_x[slot * W + w] = k.x[w];
+                a.tie_z[slot * W + w] = k.z[w];
+            }
+            a.tie_bucket[slot] = bucket_of<W>(k, a.rows, a.ref_bits);
+            a.tie_pos[slot] = pos[q];
+            a.tie_rank[slot] = u[q];
         }
-        a.tie_bucket[slot] = bucket_of<W>(k, a.rows, a.ref_bits);
-        a.tie_pos[slot] = s + r;
-        a.tie_rank[slot] = u;
     }
 }
 
@@ -405,7 +435,11 @@ hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n) {
 
 template <int W, typename T>
 hipError_t launch_topn_gather_ties(hipStream_t st, const TopnTieArgs<W, T>& a, uint32_t num_buckets) {
-    hipLaunchKernelGGL((k_topn_gather_ties<W, T>), dim3(num_buckets), dim3(kBlock), 0, st, a);
+    (void)num_buckets;
+    if (a.buckets_per_block == 0 || a.buckets_per_block > kHistBucketsMax) return hipErrorInvalidValue;
+    const uint32_t blocks = (a.num_buckets + a.buckets_per_block - 1) / a.buckets_per_block;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_topn_gather_ties<W, T>), dim3(blocks), dim3(kBlock), 0, st, a);
     return hipGetLastError();
 }
 

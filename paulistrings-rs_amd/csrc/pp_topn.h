@@ -46,6 +46,10 @@ constexpr int kTopnDigitBits = 11;
 constexpr int kTopnBins = 1 << kTopnDigitBits;
 constexpr int kTopnPasses = 6;  // digits at shifts 53, 42, 31, 20, 9 (11 bits) and 0 (9 bits)
 constexpr uint32_t kHistBucketsMax = 256;  // buckets per histogram workgroup (one length per thread)
+// Copies of the global histogram: workgroup w adds its counts to copy w % kHistCopies, so that the ~2048 workgroups'
+// same-address atomics on a few occupied bins spread over kHistCopies addresses; the digit select sums the copies.
+constexpr uint32_t kHistCopies = 4;
+constexpr uint32_t kHistWords = kHistCopies * kTopnBins;
 
 __host__ __device__ inline int topn_shift(int pass) { return pass < 5 ? 53 - 11 * pass : 0; }
 __host__ __device__ inline int topn_bits(int pass) { return pass < 5 ? 11 : 9; }
@@ -60,7 +64,8 @@ struct TopnHistArgs {
     uint32_t buckets_per_block;
     int32_t pass;
     const TopnState* state;
-    unsigned long long* hist;  // kTopnBins, zero on entry (k_topn_select clears it after each pass)
+    unsigned long long* hist;  // kHistWords (kHistCopies copies of kTopnBins), zero on entry (k_topn_select clears
+                               // it after each pass)
 };
 
 template <int W, typename T>

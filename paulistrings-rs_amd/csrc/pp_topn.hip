@@ -51,7 +51,11 @@ __device__ __forceinline__ void select_digit(TopnState* s, unsigned long long* h
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
         const int d = t * kPerThread + j;
-        mine[j] = d < bins ? hist[d] : 0ull;
+        mine[j] = 0ull;
+        if (d < bins) {
+#pragma unroll
+            for (uint32_t c = 0; c < kHistCopies; ++c) mine[j] += hist[c * kTopnBins + d];
+        }
         sum += mine[j];
     }
     suffix[t] = sum;
@@ -103,7 +107,7 @@ __device__ __forceinline__ void select_digit(TopnState* s, unsigned long long* h
         }
     }
     __syncthreads();
-    for (int i = t; i < kTopnBins; i += kBlock) hist[i] = 0;
+    for (uint32_t i = t; i < kHistWords; i += kBlock) hist[i] = 0;
 }
 
 // One radix pass: each workgroup histograms the digit of its buckets' terms that match the prefix found so far and
@@ -189,8 +193,9 @@ __global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
         }
     }
     __syncthreads();
+    unsigned long long* const copy = a.hist + (blockIdx.x % kHistCopies) * kTopnBins;
     for (int i = threadIdx.x; i < kTopnBins; i += kBlock)
-        if (h[i]) atomicAdd(&a.hist[i], (unsigned long long)h[i]);
+        if (h[i]) atomicAdd(&copy[i], (unsigned long long)h[i]);
 }
 
 // One workgroup after each histogram pass (a kernel boundary, not a device-scope fence per histogram workgroup: on

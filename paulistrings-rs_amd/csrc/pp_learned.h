@@ -40,8 +40,11 @@ struct MlpBatch {
 
 hipError_t launch_mlp_refresh(hipStream_t st, const float* params, uint16_t* w1t, uint16_t* w2t, uint16_t* w2);
 hipError_t launch_mlp_predict(hipStream_t st, const MlpDev& m, const MlpBatch& b, float* out);
-// grads (kMlpParams floats) and loss_sum (1 double) accumulate: clear them first.
-hipError_t launch_mlp_grad(hipStream_t st, const MlpDev& m, const MlpBatch& b, float* grads, double* loss_sum);
+// The gradient of the batch's mean squared error into grads (kMlpParams floats) and the sum of its squared errors
+// into loss[0], through per-tile partials (partial: partial_tiles x kMlpParams floats, loss_part: partial_tiles
+// doubles, at least one tile per kMlpTile rows) summed in a fixed order.
+hipError_t launch_mlp_grad(hipStream_t st, const MlpDev& m, const MlpBatch& b, float* partial, double* loss_part,
+                           uint64_t partial_tiles, float* grads, double* loss);
 hipError_t launch_mlp_adam(hipStream_t st, float* params, float* adam_m, float* adam_v, const float* grads,
                            float lr, float beta1, float beta2, float eps, int step);
 

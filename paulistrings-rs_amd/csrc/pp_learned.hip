@@ -277,20 +277,25 @@ struct GradLds {
     uint16_t dzt[kMlpHid * kMlpTile];    // bf16 dz2, later dz1, transposed [hidden][rows]
     float h[kMlpTile * kMlpHid];         // fp32 h2, then dz2, then dz1
     float dy[kMlpTile];
+    double e2[kMlpTile];                 // each row's squared error
 };
 
-// Column sums over the tile's 64 rows of an fp32 [64][128] matrix, added to g[0 .. 128).
-__device__ __forceinline__ void add_column_sums(const float* h, float* g) {
+// Column sums over the tile's 64 rows of an fp32 [64][128] matrix, stored to g[0 .. 128).
+__device__ __forceinline__ void store_column_sums(const float* h, float* g) {
     if (threadIdx.x < kMlpHid) {
         float s = 0.f;
         for (int r = 0; r < kMlpTile; ++r) s += h[r * kMlpHid + threadIdx.x];
-        atomicAdd(&g[threadIdx.x], s);
+        g[threadIdx.x] = s;
     }
 }
 
-__global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch b, float* __restrict__ grads,
-                                                  double* __restrict__ loss_sum) {
+// Workgroup t writes the gradient of its 64 rows to partial[t * kMlpParams ...] (every parameter but W1's padding
+// rows, each by one thread) and their squared error to loss_part[t]; k_mlp_reduce sums the tiles in tile order, so
+// that a gradient, and the training built on it, is the same on every run.
+__global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch b, float* __restrict__ partial,
+                                                  double* __restrict__ loss_part) {
     __shared__ GradLds s;
+    float* const grads = partial + (uint64_t)blockIdx.x * kMlpParams;
     const FwdWeights fw = load_fwd_weights(m, s.f);
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t base = (uint64_t)blockIdx.x * kMlpTile;
@@ -307,12 +312,14 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch
     if (threadIdx.x < kMlpTile) {
         const uint64_t i = base + threadIdx.x;
         float d = 0.f;
+        double e2 = 0.0;
         if (i < b.count) {
             const float e = s.f.y[threadIdx.x] - b.label[i];
             d = 2.f * e / (float)b.count;
-            atomicAdd(loss_sum, (double)e * (double)e);
+            e2 = (double)e * (double)e;
         }
         s.dy[threadIdx.x] = d;
+        s.e2[threadIdx.x] = e2;
     }
     // transposes of a0 and a1 for the weight-gradient GEMMs
     for (int i = threadIdx.x; i < kMlpTile * kMlpInPad; i += 256) {
@@ -329,11 +336,15 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch
     if (threadIdx.x < kMlpHid) {
         float g = 0.f;
         for (int r = 0; r < kMlpTile; ++r) g += s.h[r * kMlpHid + threadIdx.x] * s.dy[r];
-        atomicAdd(&grads[kOffW3 + threadIdx.x], g);
+        grads[kOffW3 + threadIdx.x] = g;
     } else if (threadIdx.x == kMlpHid) {
         float g = 0.f;
         for (int r = 0; r < kMlpTile; ++r) g += s.dy[r];
-        atomicAdd(&grads[kOffB3], g);
+        grads[kOffB3] = g;
+    } else if (threadIdx.x == kMlpHid + 1) {
+        double l = 0.0;
+        for (int r = 0; r < kMlpTile; ++r) l += s.e2[r];
+        loss_part[blockIdx.x] = l;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kMlpTile * kMlpHid; i += 256) {
@@ -345,7 +356,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch
         s.dzt[j * kMlpTile + r] = bd;
     }
     __syncthreads();
-    add_column_sums(s.h, grads + kOffB2);
+    store_column_sums(s.h, grads + kOffB2);
     // dW2[i][o] = sum_m a1[m][i] dz2[m][o]: D[o][i] = dz2t[o][.] . a1t[i][.], 64 tiles of 16x16, K = 64 rows
     for (int t = w; t < 64; t += 4) {
         const int ot = t >> 3, it = t & 7;
@@ -358,7 +369,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int o = 16 * ot + 4 * (l >> 4) + q;
-            atomicAdd(&grads[kOffW2 + i * kMlpHid + o], acc[q]);
+            grads[kOffW2 + i * kMlpHid + o] = acc[q];
         }
     }
     __syncthreads();
@@ -379,7 +390,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch
         }
     }
     __syncthreads();
-    add_column_sums(s.h, grads + kOffB1);
+    store_column_sums(s.h, grads + kOffB1);
     // dW1[k][o] = sum_m a0[m][k] dz1[m][o]: D[o][k] = dz1t[o][.] . a0t[k][.], 8 x 4 tiles (inputs >= 35 skipped)
     for (int t = w; t < 32; t += 4) {
         const int ot = t >> 2, kt = t & 3;
@@ -393,10 +404,27 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const MlpDev m, const MlpBatch
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int o = 16 * ot + 4 * (l >> 4) + q;
-                atomicAdd(&grads[kOffW1 + k * kMlpHid + o], acc[q]);
+                grads[kOffW1 + k * kMlpHid + o] = acc[q];
             }
         }
     }
+}
+
+// grads[i] = the tiles' partial gradients summed in tile order (W1's padding rows zero); loss[0] likewise.
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float* __restrict__ partial, uint32_t tiles,
+                                                    float* __restrict__ grads, const double* __restrict__ loss_part,
+                                                    double* __restrict__ loss) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i == 0) {
+        double l = 0.0;
+        for (uint32_t t = 0; t < tiles; ++t) l += loss_part[t];
+        loss[0] = l;
+    }
+    if (i >= kMlpParams) return;
+    float g = 0.f;
+    if (!(i >= kOffW1 + kMlpIn * kMlpHid && i < kOffB1))
+        for (uint32_t t = 0; t < tiles; ++t) g += partial[(uint64_t)t * kMlpParams + i];
+    grads[i] = g;
 }
 
 __global__ void __launch_bounds__(256) k_mlp_adam(float* __restrict__ p, float* __restrict__ am,
@@ -511,10 +539,14 @@ hipError_t launch_mlp_predict(hipStream_t st, const MlpDev& m, const MlpBatch& b
     return hipGetLastError();
 }
 
-hipError_t launch_mlp_grad(hipStream_t st, const MlpDev& m, const MlpBatch& b, float* grads, double* loss_sum) {
-    if (b.count == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mlp_grad, dim3((uint32_t)((b.count + kMlpTile - 1) / kMlpTile)), dim3(256), 0, st, m, b,
-                       grads, loss_sum);
+hipError_t launch_mlp_grad(hipStream_t st, const MlpDev& m, const MlpBatch& b, float* partial, double* loss_part,
+                           uint64_t partial_tiles, float* grads, double* loss) {
+    const uint64_t tiles = (b.count + kMlpTile - 1) / kMlpTile;
+    if (tiles > partial_tiles) return hipErrorInvalidValue;
+    if (tiles)
+        hipLaunchKernelGGL(k_mlp_grad, dim3((uint32_t)tiles), dim3(256), 0, st, m, b, partial, loss_part);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, partial, (uint32_t)tiles,
+                       grads, loss_part, loss);
     return hipGetLastError();
 }
 

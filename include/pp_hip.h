@@ -199,10 +199,12 @@ pp_status pp_model_set_params(pp_model* model, const float* params);  /* also re
 pp_status pp_model_get_params(pp_model* model, float* params);
 pp_status pp_model_predict(pp_model* model, const uint64_t* x, const uint64_t* z, const double* coeff,
                            const float* remaining, uint64_t n, float* out);
-/* Gradient of the mean squared error of the predictions against labels, and that error. */
+/* Gradient of the mean squared error of the predictions against labels, and that error.  Both are summed in a
+ * fixed order (per 64-sample tile, then over tiles): the same inputs give the same bits on every call. */
 pp_status pp_model_gradient(pp_model* model, const uint64_t* x, const uint64_t* z, const double* coeff,
                             const float* remaining, const float* labels, uint64_t n, float* grads, double* loss);
-/* Adam (0.9, 0.999, 1e-8) over `epochs` shuffled passes in batches of `batch`; loss = the last epoch's mean. */
+/* Adam (0.9, 0.999, 1e-8) over `epochs` shuffled passes in batches of `batch`; loss = the last epoch's mean.
+ * Deterministic for a given seed (the shuffle is a splitmix64 stream, the gradients as pp_model_gradient's). */
 pp_status pp_model_train(pp_model* model, const uint64_t* x, const uint64_t* z, const double* coeff,
                          const float* remaining, const float* labels, uint64_t n, uint32_t epochs, uint32_t batch,
                          float lr, uint64_t seed, double* loss);

@@ -88,6 +88,26 @@ def test_training_learns_a_teacher():
     m.free()
 
 
+def test_training_is_deterministic():
+    """The same samples, initialization and shuffle seed give the same weights bit for bit: gradients are summed per
+    64-sample tile and then over tiles in a fixed order (a batch of 512 spans 8 tiles), never by float atomics."""
+    announce("mlp training determinism")
+    rng = np.random.default_rng(6)
+    groups = L.lattice_groups(8, 8)
+    x, z, c, rem = samples(rng, 3000)
+    labels = rng.standard_normal(3000).astype(np.float32)
+    out = []
+    for _ in range(2):
+        m = PL.Model(groups, depth_scale=40.0, seed=7)
+        loss = m.train(x, z, c, rem, labels, epochs=5, batch=512, lr=1e-3, seed=8)
+        g, gl = m.gradient(x, z, c, rem, labels)
+        out.append((m.get_params(), loss, g, gl))
+        m.free()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and out[0][3] == out[1][3]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
 @pytest.mark.parametrize("n", [1, 500, 3000])
 def test_learned_pass_keeps_the_top_scores(n):
     announce(f"learned pass n={n}")

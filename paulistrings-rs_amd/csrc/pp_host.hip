@@ -305,6 +305,7 @@ struct pp_sum {
     bool pending_topn = false;
     uint64_t phys_len = 0;
     const uint8_t* pending_flags = nullptr;
+    const float* pending_score = nullptr;  // the ranking score when the selection was by score
     std::vector<uint64_t> tie_kept;  // TopN's kept boundary positions, the source of an asynchronous copy
     uint64_t* d_tiles = nullptr;  // scratch of the bucket-offset scan
     uint64_t tiles_cap = 0;
@@ -763,6 +764,7 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         a.count_totals = count_in_join ? 1u : 0u;
         a.filter = src->pending_topn ? src->d_topn : nullptr;
         a.filter_flags = src->pending_topn ? src->pending_flags : nullptr;
+        a.filter_score = src->pending_topn ? src->pending_score : nullptr;
         hipError_t le = launch_rotation_join<W, T>(ctx->stream, a, ctx->tuning.groups_per_cu, ctx->tuning.join_block);
         if (le != hipSuccess || scap >= lcap) return le;  // a first launch staging lcap never queues a pair
         a.redo_mode = 1;

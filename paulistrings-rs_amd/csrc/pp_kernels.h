@@ -45,9 +45,10 @@ struct JoinArgs {
     uint32_t count_totals;           // each pair adds its rows to stats[0] and maxes stats[1] (few buckets: no
                                      // k_reduce_lens launch)
     // A finished TopN selection not yet applied to the input (null: none): rows it drops are not read further, as
-    // though its retain had run (|c| ranking only).
+    // though its retain had run.
     const TopnState* filter;
     const uint8_t* filter_flags;  // its boundary-bin flags per input position, or null
+    const float* filter_score;    // its ranking score per input position when it ranked by score, else null (|c|)
 };
 
 template <int W, typename T>

@@ -454,8 +454,9 @@ __device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, 
             bool valid = r < n;
             const uint32_t j = r >= m.n0 ? 1u : 0u;
             if (valid && a.filter) {  // uniform test: a pending TopN retain
-                const uint64_t u = magnitude_bits<T>(cb.c);
-                valid = topn_keeps(fs, u, is_nan_bits(u), a.filter_flags, j ? m.s1 + (r - m.n0) : m.s0 + r);
+                const uint64_t i = j ? m.s1 + (r - m.n0) : m.s0 + r;
+                const uint64_t u = a.filter_score ? score_bits(a.filter_score[i]) : magnitude_bits<T>(cb.c);
+                valid = topn_keeps(fs, u, !a.filter_score && is_nan_bits(u), a.filter_flags, i);
             }
             bool anti = false, keep = false;
             if (valid) {

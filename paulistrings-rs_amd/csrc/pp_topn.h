@@ -29,6 +29,15 @@ __device__ __forceinline__ uint64_t magnitude_bits(T c) {
     return (uint64_t)__double_as_longlong(v * v);
 }
 
+// A float score as a key in the high half of a u64 that orders as the score (a NaN score sorts lowest): the learned
+// TopN pass ranks terms by their score through the same selection as TopN.
+__device__ __forceinline__ uint64_t score_bits(float s) {
+    if (s != s) return 0ull;
+    const uint32_t u = __float_as_uint(s);
+    const uint32_t k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return (uint64_t)k << 32;
+}
+
 // Whether a term whose ranking key is u (at column position i) survives a finished selection: above the boundary
 // bin on the decided bits, or in it and flagged (without flags, every term in it); with at most n finite terms every
 // finite one; a NaN |c|^2 never (a score key is never NaN-coded: score_nan marks the |c| case).

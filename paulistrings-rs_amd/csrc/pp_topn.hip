@@ -9,15 +9,6 @@
 
 namespace pp {
 
-// A float score as a key in the high half of a u64 that orders as the score (a NaN score sorts lowest): the learned
-// TopN pass ranks terms by their score through the same selection as TopN.
-__device__ __forceinline__ uint64_t score_bits(float s) {
-    if (s != s) return 0ull;
-    const uint32_t u = __float_as_uint(s);
-    const uint32_t k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-    return (uint64_t)k << 32;
-}
-
 template <typename T>
 __device__ __forceinline__ uint64_t rank_bits(const T* c, const float* score, uint64_t i) {
     return score ? score_bits(score[i]) : magnitude_bits<T>(c[i]);

@@ -1709,7 +1709,8 @@ pp_status batch_step(pp_batch* b, const std::vector<uint32_t>& idx, const std::v
         pp_sum* s = b->sums[idx[k]];
         const int in = s->cur, out = 1 - s->cur;
         const uint32_t B = 1u << s->device_bits;
-        if ((st = ensure_columns(s, out, 2 * s->len + (uint64_t)kSegAlign * B)) != PP_OK) return st;
+        const uint64_t rows_held = s->pending_topn ? s->phys_len : s->len;
+        if ((st = ensure_columns(s, out, 2 * rows_held + (uint64_t)kSegAlign * B)) != PP_OK) return st;
         if ((st = ensure_buckets(s, out, B)) != PP_OK) return st;
         const pp_channel& ch = chans[k];
         const uint32_t delta = bucket_of_host(s, ch.gx, ch.gz, s->device_bits);
@@ -1749,6 +1750,9 @@ pp_status batch_step(pp_batch* b, const std::vector<uint32_t>& idx, const std::v
         a.keep = keeps[k];
         a.stats = s->d_stats;
         a.redo = nullptr;  // every item stages its largest pair
+        a.filter = s->pending_topn ? s->d_topn : nullptr;  // a TopN whose retain this join performs
+        a.filter_flags = s->pending_topn ? s->pending_flags : nullptr;
+        a.filter_score = s->pending_topn ? s->pending_score : nullptr;
         items[k] = a;
         b->h_pair_start[k] = total_pairs;
         total_pairs += a.num_cosets;
@@ -1768,6 +1772,7 @@ pp_status batch_step(pp_batch* b, const std::vector<uint32_t>& idx, const std::v
     for (size_t k = 0; k < idx.size(); ++k) {
         pp_sum* s = b->sums[idx[k]];
         if (b->h_out[4 * k + 2]) return fail(ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
+        s->pending_topn = false;
         s->len = b->h_out[4 * k];
         s->max_bucket = (uint32_t)b->h_out[4 * k + 1];
         s->cur = 1 - s->cur;
@@ -1818,7 +1823,8 @@ pp_status pp_batch_propagate(pp_batch* b, const pp_channel* channels, const uint
             const Prepared p = prepare(s, ch, plan.adjoint);
             if (p.key_preserving) {  // in place, on its own
                 if ((st = device_layer(s, ch, p, keeps[i])) != PP_OK) return st;
-                if (++next[i] == run_end[i] && (st = plan.finish_run(s)) != PP_OK) return st;
+                if (++next[i] == run_end[i] && (st = plan.finish_run(s, next[i] < plan.n_channels)) != PP_OK)
+                    return st;
                 continue;
             }
             idx.push_back(i);
@@ -1838,7 +1844,9 @@ pp_status pp_batch_propagate(pp_batch* b, const pp_channel* channels, const uint
                                                  : batch_step<2, float>(b, idx, chans, preps, ks));
         if (st != PP_OK) return st;
         for (uint32_t i : idx)
-            if (++next[i] == run_end[i] && (st = plans[i].finish_run(b->sums[i])) != PP_OK) return st;
+            if (++next[i] == run_end[i] &&
+                (st = plans[i].finish_run(b->sums[i], next[i] < plans[i].n_channels)) != PP_OK)
+                return st;
     }
     return PP_OK;
 }

@@ -306,6 +306,28 @@ def test_topn_every_run_with_every_layer_kind(num_qubits, dtype):
     assert_same(o, p)
 
 
+@pytest.mark.parametrize("topn", [1500, 100000])
+def test_nan_coefficients_through_joins_and_topn(topn):
+    """NaN coefficients (5% of the input, no threshold): the joins carry them as IEEE arithmetic does, TopN never keeps
+    one (with its retain left to the next join), and with TopN inactive they reach the output: keys, values (NaN
+    equal to NaN) and every layer's length against the oracle."""
+    announce(f"nan coefficients topn={topn}")
+    rng = np.random.default_rng(91)
+    x, z, c = random_sum(rng, 2000, 30, 0.15, F64)
+    c[rng.random(2000) < 0.05] = np.nan
+    circ = random_circuit(rng, 30, 20)
+    s = po.OracleSum(30, x, z, c)
+    olens = s.propagate(circ.channels(), "heisenberg", oracle_policy(topn=topn))
+    ox, oz, oc, _ = s.export()
+    host = PauliSum(30, x, z, c)
+    stats = host.propagate_with_stats(circ, product_policy(topn=topn), "heisenberg")
+    assert np.array_equal(host.x_array(), ox) and np.array_equal(host.z_array(), oz)
+    np.testing.assert_array_equal(host.coefficients_array(), oc)
+    assert [int(v) for v in olens] == stats.terms_out
+    if topn == 100000:
+        assert np.isnan(oc).any()
+
+
 @pytest.mark.parametrize("case", ["normal_w2_f64", "ties_w1_f64", "normal_w2_f32", "big_w2_f64"])
 def test_approx_topn_standalone(case):
     announce(f"approx topn {case}")

@@ -132,20 +132,32 @@ def test_learned_pass_keeps_the_top_scores(n):
     assert np.array_equal(got.coefficients_array(), hc[keep])
 
 
-def test_propagate_learned_matches_a_layer_by_layer_emulation():
+@pytest.mark.parametrize("clifford_block", [False, True])
+def test_propagate_learned_matches_a_layer_by_layer_emulation(clifford_block):
     """pp_propagate_learned against the oracle stepped one channel at a time, with the learned pass emulated on the
     host: after every layer that leaves more than `keep` terms, keep the `keep` of largest score, the score being the
     device network's prediction for the oracle's terms (pp_model_predict, the same per-term arithmetic as the pass's
     score kernel) plus ln|c| in fp32.  While a run's worst-case growth stays within `keep` the device joins layers
     into runs without a pass, and then no layer of the run exceeds `keep` either, so the two agree layer by layer.
-    Kept keys and coefficients must be equal bit for bit."""
+    Kept keys and coefficients must be equal bit for bit.  With clifford_block, a block of 24 quarter-turn ZZ layers
+    between the two steps forms a permutation run, walked in one pass on the device."""
     import pp_oracle as po
     from paulistrings_amd import truncation
     from paulistrings_amd.circuits import repeat, tfim_square_step
 
-    announce("propagate_learned against a layer-by-layer emulation")
+    announce(f"propagate_learned against a layer-by-layer emulation clifford_block={clifford_block}")
     n_q, keep, eps = 16, 300, 1e-7
-    circ = repeat(tfim_square_step(4, 4, dt=0.1, j=1.0, h=2.0), 2)
+    step = tfim_square_step(4, 4, dt=0.1, j=1.0, h=2.0)
+    circ = repeat(step, 2)
+    if clifford_block:
+        from paulistrings_amd import Circuit
+        block = Circuit(n_q)
+        for q in range(n_q):
+            if q % 4 != 3:
+                block.rzz(np.pi / 2 if q % 2 else -np.pi / 2, q, q + 1)
+            if q < 12:
+                block.rzz(np.pi / 2, q, q + 4)
+        circ = Circuit(n_q).extend(step).extend(block).extend(step)
     chans = circ.channels()
     m = PL.Model(PL.lattice_groups(4, 4), depth_scale=48.0, seed=21)
     x0 = (np.uint64(1) << np.arange(n_q, dtype=np.uint64)).reshape(-1, 1)

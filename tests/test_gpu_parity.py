@@ -306,6 +306,18 @@ def test_topn_every_run_with_every_layer_kind(num_qubits, dtype):
     assert_same(o, p)
 
 
+@pytest.mark.parametrize("topn", [0, 1, 2999, 3000])
+def test_topn_at_its_edges_inside_a_propagation(topn):
+    """TopN of 0 (clears the sum after the first run), 1, and just below and at the input size, inside a propagation
+    (each selection's retain left to the next join): keys, coefficients and every layer's length."""
+    announce(f"topn edge {topn}")
+    rng = np.random.default_rng(97)
+    x, z, c = random_sum(rng, 3000, 50, 0.1, F64)
+    circ = random_circuit(rng, 50, 12)
+    o, p = run_both(50, x, z, c, circ, "heisenberg", eps=1e-5, topn=topn)
+    assert_same(o, p)
+
+
 @pytest.mark.parametrize("topn", [1500, 100000])
 def test_nan_coefficients_through_joins_and_topn(topn):
     """NaN coefficients (5% of the input, no threshold): the joins carry them as IEEE arithmetic does, TopN never keeps

@@ -467,9 +467,16 @@ def kicked_ising_line(cpu_budget_s: float) -> dict:
         lens = o.propagate(chans[:per_step], "heisenberg", opol, threads=threads)
         spent += time.perf_counter() - t1
         cpu_rows += int(lens.sum())
+        # the oracle's per-layer lengths must be the GPU run's, layer for layer
+        want = st.terms_out[k * per_step:(k + 1) * per_step]
+        if [int(v) for v in lens] != want:
+            bad = next(i for i, (a, b) in enumerate(zip(lens, want)) if int(a) != b)
+            raise AssertionError(f"kicked ising: step {k} layer {bad}: oracle {int(lens[bad])} terms, "
+                                 f"GPU {want[bad]}")
         done += 1
         if spent > cpu_budget_s:
             break
+    log(f"kicked ising: the oracle's per-layer lengths equal the GPU's for the first {done} steps")
     return {
         "metric": "merged terms/s per layer (whole propagation)",
         "value": rows_out / gpu_s,
@@ -489,7 +496,8 @@ def kicked_ising_line(cpu_budget_s: float) -> dict:
         "cpu_baseline": {"value": cpu_rows / spent, "unit": "terms/s", "cores": threads, "kind": "port",
                          "sample": f"the oracle on the first {done} of the 20 steps ({spent:.1f} s)"},
         "detail": {"layers": len(st.terms_out), "output_terms": rows_out, "peak_terms": st.peak_terms,
-                   "final_terms": st.final_terms, "readout": v},
+                   "final_terms": st.final_terms, "readout": v,
+                   "oracle_layer_lengths_equal_for_steps": done},
     }
 
 

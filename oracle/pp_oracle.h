@@ -25,8 +25,9 @@
  *   ppo_mix64 / ppo_splitmix64 ... the reference's known answer, state 0 -> 0xE220A8397B1DCDAF (rng/tests.rs:3-8)
  *   ppo_hash_rows ................ at 127 qubits the rows are the attempt-0 draws (the reference's own property,
  *                                  pauli_sum/hash/tests.rs:209-221), plus full rank, width independence, linearity;
- *                                  parity unpinned: the rows at qubit counts where rank completion redraws
- *                                  (never at 127 qubits); they only move bucket ids (TopN tie order, export order)
+ *                                  at 1..32, 36 and 64 qubits and four seeds the exact rows, redrawn ones included,
+ *                                  equal an independent restatement of row_word and accept_rows with its own GF(2)
+ *                                  elimination (pauli_sum/hash.rs:22-26, :102-130)
  *   ppo_desired_bits / ppo_rebucketed_bits ... hand-computed values of the storage.rs:15-41 formulas
  *   ppo_sin_cos .................. quarter-turn snaps (theta = -pi/2 gives exactly -YZ); the ulp-level values
  *                                  are glibc's sincos, the library boundary the reference also crosses:

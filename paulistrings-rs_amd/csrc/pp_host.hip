@@ -240,11 +240,27 @@ void snapped_sin_cos(double theta, double* s, double* c) {
 // Context and sum
 // ---------------------------------------------------------------------------------------------------------------
 
+// What one captured out-of-place layer (pp_apply_rotation_to inside a graph capture) baked into the graph: the two
+// sums, the column sets and device buffers its kernels read and write, and the input bounds its launch was sized for.
+// A replay is refused unless every one of them is still what the sums hold.
+struct CapturedJoin {
+    pp_sum* src;
+    pp_sum* dst;
+    int src_cur, dst_cur, device_bits;
+    const void* src_ptrs[5];  // x, z, c, bucket start, bucket len
+    const void* dst_ptrs[7];  // x, z, c, bucket start, bucket len, counters, redo queue
+    uint64_t dst_cap;         // rows of dst's written column set
+    uint64_t src_len;         // rows src held at capture: the output columns and staging were sized from them
+    uint32_t src_max_bucket;
+};
+
 struct pp_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string error;
     bool capturing = false;  // between pp_graph_capture_begin and _end: no allocation, no synchronization
+    std::vector<CapturedJoin> capture_log;  // the layers of the open capture
+    std::vector<pp_graph*> graphs;          // live graphs of this context (pp_sum_free refuses their sums)
     pp_tuning tuning{};      // sizes of the device path (pp_set_tuning, pp_autotune)
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -252,6 +268,13 @@ struct pp_context {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> score_events;  // the learned pass's score kernel
     uint64_t prof_scored = 0;
     unsigned long long* d_prof = nullptr;  // [0]: output rows of the profiled joins
+};
+
+struct pp_graph {
+    pp_context* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<CapturedJoin> layers;  // what the replay reads and writes, checked before every launch
 };
 
 struct Columns {
@@ -1236,6 +1259,12 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
 pp_status pp_sum_free(pp_sum* s) {
     if (s) PP_NOT_CAPTURING(s->ctx, "pp_sum_free");
     if (!s) return PP_OK;
+    if (s->ctx)
+        for (const pp_graph* g : s->ctx->graphs)
+            for (const CapturedJoin& r : g->layers)
+                if (r.src == s || r.dst == s)
+                    return fail(s->ctx, PP_ERR_INVALID_ARGUMENT,
+                                "pp_sum_free: a captured graph reads or writes this sum; free the graph first");
     if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
     for (int i = 0; i < 2; ++i) {
         free_columns(s->col[i]);
@@ -1448,18 +1477,43 @@ pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* ch, i
         return fail(ctx, PP_ERR_INVALID_ARGUMENT, "pp_apply_rotation_to: the sums differ in shape or context");
     if (!valid_policy(policy)) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
     pp_status st;
-    if (ctx->capturing && src->stale)
-        return fail(ctx, PP_ERR_UNSUPPORTED, "pp_apply_rotation_to: the input's counts are pending during capture");
+    if (ctx->capturing && (src->stale || src->pending_topn))
+        return fail(ctx, PP_ERR_UNSUPPORTED,
+                    "pp_apply_rotation_to: the input's counts or a selection are pending during capture");
     if ((st = refresh(src)) != PP_OK) return st;
     if ((st = settle_device_bits(src)) != PP_OK) return st;
     const Prepared p = prepare(src, *ch, adjoint != 0);
     if (p.key_preserving)
         return fail(ctx, PP_ERR_UNSUPPORTED, "pp_apply_rotation_to: a key-preserving channel runs in place");
+    // dst is overwritten whole: a selection it still had pending no longer describes its rows
+    dst->pending_topn = false;
+    dst->pending_flags = nullptr;
+    dst->pending_score = nullptr;
+    dst->phys_len = 0;
     dst->stale = false;
     dst->device_bits = src->device_bits;
     dst->reference_bits = src->reference_bits;
-    if ((st = launch_join(src, dst, dst->cur, *ch, p, lower_policy(policy))) != PP_OK) return st;
+    const int out = dst->cur;
+    if ((st = launch_join(src, dst, out, *ch, p, lower_policy(policy))) != PP_OK) return st;
     dst->stale = true;
+    if (ctx->capturing) {
+        const int in = src->cur;
+        CapturedJoin r{};
+        r.src = src;
+        r.dst = dst;
+        r.src_cur = in;
+        r.dst_cur = out;
+        r.device_bits = src->device_bits;
+        const void* sp[5] = {src->col[in].x, src->col[in].z, src->col[in].c, src->bk[in].start, src->bk[in].len};
+        const void* dp[7] = {dst->col[out].x, dst->col[out].z, dst->col[out].c, dst->bk[out].start,
+                             dst->bk[out].len, dst->d_stats, dst->d_redo};
+        std::copy(sp, sp + 5, r.src_ptrs);
+        std::copy(dp, dp + 7, r.dst_ptrs);
+        r.dst_cap = dst->col[out].cap;
+        r.src_len = src->len;
+        r.src_max_bucket = src->max_bucket;
+        ctx->capture_log.push_back(r);
+    }
     return PP_OK;
 }
 
@@ -1932,11 +1986,40 @@ pp_status pp_autotune(pp_sum* sample, const pp_channel* channels, uint64_t n_cha
 
 // ---- layer sequences captured into a HIP graph ----------------------------------------------------------------
 
-struct pp_graph {
-    pp_context* ctx = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
+namespace {
+
+// Whether the sum's current state still matches what a captured layer baked in; `what` names the first mismatch.
+bool captured_join_valid(const CapturedJoin& r, std::string& what) {
+    const pp_sum* src = r.src;
+    const pp_sum* dst = r.dst;
+    const int in = r.src_cur, out = r.dst_cur;
+    const void* sp[5] = {src->col[in].x, src->col[in].z, src->col[in].c, src->bk[in].start, src->bk[in].len};
+    const void* dp[7] = {dst->col[out].x, dst->col[out].z, dst->col[out].c, dst->bk[out].start, dst->bk[out].len,
+                         dst->d_stats, dst->d_redo};
+    if (src->cur != in || src->device_bits != r.device_bits || !std::equal(sp, sp + 5, r.src_ptrs)) {
+        what = "the input's columns or device bucketing changed since the capture";
+        return false;
+    }
+    if (dst->cur != out || dst->col[out].cap != r.dst_cap || !std::equal(dp, dp + 7, r.dst_ptrs)) {
+        what = "the output's current columns changed since the capture (a layer in place, or a reallocation)";
+        return false;
+    }
+    if (src->pending_topn) {
+        what = "the input has a pending selection";
+        return false;
+    }
+    // the captured launch sized the output segments, the staging and the table from these bounds
+    const uint64_t B = 1ull << src->device_bits;
+    if (2 * src->len + (uint64_t)kSegAlign * B > r.dst_cap || src->max_bucket > r.src_max_bucket) {
+        what = "the input holds more rows (" + std::to_string(src->len) + ", longest bucket " +
+               std::to_string(src->max_bucket) + ") than the captured launch was sized for (" +
+               std::to_string(r.src_len) + ", " + std::to_string(r.src_max_bucket) + ")";
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
 
 pp_status pp_graph_capture_begin(pp_context* ctx) {
     if (!ctx) return PP_ERR_INVALID_ARGUMENT;
@@ -1944,6 +2027,7 @@ pp_status pp_graph_capture_begin(pp_context* ctx) {
     PP_HIP(ctx, hipSetDevice(ctx->device));
     PP_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
     ctx->capturing = true;
+    ctx->capture_log.clear();
     return PP_OK;
 }
 
@@ -1973,6 +2057,8 @@ pp_status pp_graph_capture_end(pp_context* ctx, pp_graph** out) {
     g->ctx = ctx;
     g->graph = graph;
     g->exec = exec;
+    g->layers.swap(ctx->capture_log);
+    ctx->graphs.push_back(g);
     *out = g;
     return PP_OK;
 }
@@ -1980,7 +2066,24 @@ pp_status pp_graph_capture_end(pp_context* ctx, pp_graph** out) {
 pp_status pp_graph_launch(pp_graph* g) {
     if (g) PP_NOT_CAPTURING(g->ctx, "pp_graph_launch");
     if (!g) return PP_ERR_INVALID_ARGUMENT;
-    PP_HIP(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream));
+    pp_context* ctx = g->ctx;
+    // Every input's counts, then every captured buffer and bound, before anything is queued: a refused replay
+    // writes nothing.  An input that is also an earlier replay's output is read after that replay's counters land.
+    for (const CapturedJoin& r : g->layers) {
+        pp_status st = refresh(r.src);
+        if (st != PP_OK) return st;
+    }
+    for (size_t i = 0; i < g->layers.size(); ++i) {
+        std::string what;
+        if (!captured_join_valid(g->layers[i], what))
+            return fail(ctx, PP_ERR_INVALID_ARGUMENT, "pp_graph_launch: layer " + std::to_string(i) + ": " + what);
+    }
+    PP_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    for (const CapturedJoin& r : g->layers) {
+        r.dst->stale = true;  // its counts come from this replay
+        r.dst->device_bits = r.device_bits;
+        r.dst->reference_bits = r.src->reference_bits;
+    }
     return PP_OK;
 }
 
@@ -1990,6 +2093,8 @@ pp_status pp_graph_free(pp_graph* g) {
     (void)hipStreamSynchronize(g->ctx->stream);
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
+    auto& live = g->ctx->graphs;
+    live.erase(std::remove(live.begin(), live.end(), g), live.end());
     delete g;
     return PP_OK;
 }

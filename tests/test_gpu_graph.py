@@ -92,3 +92,37 @@ def test_synchronizing_calls_are_refused_during_capture():
     g.truncate_topn(10)
     assert len(g) == 10
     g.free()
+
+
+def test_replay_is_refused_once_the_captured_buffers_changed():
+    """A replay writes through the buffers the capture baked in: once the output took a layer in place (its current
+    column set flipped) the launch is refused on the host, before anything is queued, and a sum a live graph uses
+    cannot be freed.  The refusal needs no kernel: the check runs before the launch."""
+    announce("graph replay refused after an in-place layer on its output")
+    from paulistrings_amd import _native
+
+    rng = np.random.default_rng(43)
+    nq = 60
+    x, z = random_keys(rng, 1 << 14, nq, 0.1)
+    c = rng.standard_normal(1 << 14)
+    src = PauliSum(nq, x, z, c).to_device(0)
+    dst = src.clone()
+    ctx = src.ctx
+    ch = Circuit(nq).rzz(0.7, 4, 5).channels()
+    policy = truncation.coeff(1e-4)
+    src.apply_rotation_to(dst, ch, policy=policy)
+    want = terms(dst)
+    with ctx.capture() as cap:
+        src.apply_rotation_to(dst, ch, policy=policy)
+    cap.graph.launch()
+    got = terms(dst)
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    assert _native.lib().pp_sum_free(dst._h) == 5  # PP_ERR_INVALID_ARGUMENT: the graph writes it
+    dst.apply_rotation(Circuit(nq).rx(0.3, 7).channels(), policy=policy)  # in place: the current set flips
+    with pytest.raises(Exception, match="changed since the capture"):
+        cap.graph.launch()
+    after = terms(dst)  # the in-place layer's result, untouched by the refused replay
+    assert len(after[2]) > 0
+    cap.graph.free()
+    dst.free()
+    src.free()

@@ -21,9 +21,13 @@ from paulistrings_amd.quench import training_set
 pytestmark = pytest.mark.gpu
 
 
-def test_trained_network_ranks_contributions_better_than_abs_c():
-    print("[gpu case] learned gain on held-out TFIM terms", flush=True)
-    sys.stdout.flush()
+_CACHE: dict = {}
+
+
+def held_out_scores():
+    """Train once per process and return the held-out labels, predictions and coefficients."""
+    if _CACHE:
+        return _CACHE["v"]
     step = tfim_square_step(8, 8, dt=0.05, j=1.0, h=3.0)
     x, z, c, rem, lab = training_set(step, train_steps=3, keep=1 << 14, per_checkpoint=1024, eps=1e-7,
                                      label_eps=1e-5, seed=7)
@@ -35,18 +39,32 @@ def test_trained_network_ranks_contributions_better_than_abs_c():
     m.train(x[tr], z[tr], c[tr], rem[tr], lab[tr], epochs=80, batch=256, lr=1e-3, seed=5)
     pred = m.predict(x[te], z[te], c[te], rem[te]).astype(np.float64)
     m.free()
-    y = lab[te].astype(np.float64)
+    _CACHE["v"] = (lab[tr], lab[te].astype(np.float64), pred, c[te])
+    return _CACHE["v"]
+
+
+def test_trained_network_predicts_read_out_factors_better_than_a_constant():
+    print("[gpu case] learned gain on held-out TFIM terms: squared error", flush=True)
+    sys.stdout.flush()
+    lab_tr, y, pred, _ = held_out_scores()
     mse_net = float(np.mean((pred - y) ** 2))
-    mse_const = float(np.mean((y - lab[tr].mean()) ** 2))  # the best a constant trained on the same labels does
-    contrib = np.abs(c[te]) * np.exp(y)  # |c| times the read-out factor
-    k = len(te) // 4
-    by_net = np.argsort(-(np.log(np.abs(c[te])) + pred))[:k]
-    by_c = np.argsort(-np.abs(c[te]))[:k]
+    mse_const = float(np.mean((y - lab_tr.mean()) ** 2))  # the best a constant trained on the same labels does
+    print(f"held-out terms {len(y)}: mse network {mse_net:.3f} vs constant {mse_const:.3f}", flush=True)
+    # the network explains most of what a constant (ranking by |c| alone) leaves unexplained
+    assert mse_net < 0.5 * mse_const
+
+
+@pytest.mark.xfail(strict=False, reason="measured on MI355X: the top quarter by the network keeps 0.96790 of the "
+                                        "held-out |contribution|, by |c| 0.96824; the ranking gain is not shown yet")
+def test_trained_network_ranks_contributions_better_than_abs_c():
+    print("[gpu case] learned gain on held-out TFIM terms: ranking", flush=True)
+    sys.stdout.flush()
+    _, y, pred, cte = held_out_scores()
+    contrib = np.abs(cte) * np.exp(y)  # |c| times the read-out factor
+    k = len(y) // 4
+    by_net = np.argsort(-(np.log(np.abs(cte)) + pred))[:k]
+    by_c = np.argsort(-np.abs(cte))[:k]
     share_net = float(contrib[by_net].sum() / contrib.sum())
     share_c = float(contrib[by_c].sum() / contrib.sum())
-    print(f"held-out terms {len(te)}: mse network {mse_net:.3f} vs constant {mse_const:.3f}; top quarter keeps "
-          f"{share_net:.3f} of the |contribution| by the network, {share_c:.3f} by |c|", flush=True)
-    # the gain: the network explains most of what a constant (ranking by |c| alone) leaves unexplained
-    assert mse_net < 0.5 * mse_const
-    # and its ranking keeps as much of the held-out contribution as |c| does (the large-|c| terms dominate the sum)
-    assert share_net >= 0.98 * share_c
+    print(f"top quarter keeps {share_net:.5f} of the |contribution| by the network, {share_c:.5f} by |c|", flush=True)
+    assert share_net > share_c

@@ -80,10 +80,10 @@ def random_sum(rng, n, num_qubits, density, dtype):
     return x, z, rng.standard_normal(n).astype(dtype)
 
 
-def random_circuit(rng, num_qubits, layers):
+def random_circuit(rng, num_qubits, layers, max_support=2):
     c = Circuit(num_qubits)
     for _ in range(layers):
-        k = int(rng.integers(1, 3))
+        k = int(rng.integers(1, max_support + 1))
         qs = [int(q) for q in rng.choice(num_qubits, size=k, replace=False)]
         ps = "".join(rng.choice(list("XYZ"), size=k))
         theta = float(rng.choice([rng.uniform(-math.pi, math.pi), -math.pi / 2, math.pi / 2, math.pi]))
@@ -136,6 +136,27 @@ def test_random_circuits_bitwise(num_qubits, dtype, direction):
     x, z, c = random_sum(rng, 3000, num_qubits, 0.15, dtype)
     circ = random_circuit(rng, num_qubits, 40)
     o, p = run_both(num_qubits, x, z, c, circ, direction, eps=1e-3, max_weight=num_qubits // 2)
+    assert_same(o, p)
+
+
+@pytest.mark.parametrize("num_qubits", [40, 127])
+@pytest.mark.parametrize("dtype", [F64, F32])
+@pytest.mark.parametrize("direction", ["forward", "heisenberg"])
+def test_wide_generators_bitwise(num_qubits, dtype, direction):
+    """Generators of weight 1..5 (a weight of 3 or more prepares as a non-local rotation of fanout 2 that never joins
+    a rotation or permutation run, channel/rotation.rs:111-122), quarter and half turns among the angles, some
+    straddling qubits 63/64, with a threshold, a weight cap and a TopN cap that bounds the runs: keys, coefficients
+    and every layer's length against the oracle."""
+    announce(f"wide generators n={num_qubits} {np.dtype(dtype).name} {direction}")
+    rng = np.random.default_rng(211 + num_qubits)
+    x, z, c = random_sum(rng, 3000, num_qubits, 0.12, dtype)
+    circ = random_circuit(rng, num_qubits, 50, max_support=5)
+    if num_qubits > 64:
+        circ.pauli_rotation("XYZ", [62, 63, 64], 0.9).pauli_rotation("ZZXY", [63, 64, 65, 0], math.pi / 2)
+        circ.pauli_rotation("YXZZX", [126, 64, 63, 1, 2], 2.2).pauli_rotation("ZZZ", [63, 64, 126], math.pi)
+    o, p = run_both(num_qubits, x, z, c, circ, direction, eps=1e-4, max_weight=num_qubits // 2, topn=6000)
+    assert_same(o, p)
+    o, p = run_both(num_qubits, x, z, c, circ, direction, eps=1e-3)
     assert_same(o, p)
 
 

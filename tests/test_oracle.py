@@ -72,6 +72,64 @@ def test_hash_rows_are_full_rank(n):
     assert gf2_rank(packed) == len(rx)
 
 
+def independent_hash_rows(num_qubits: int, seed: int):
+    """Gf2Hash::new's rows from their published rule (pauli_sum/hash.rs:22-26 row_word, :102-130 accept_rows), with
+    a GF(2) elimination of its own: row r takes the first draw (attempt 0, 1, ... up to 4096) that is independent of
+    the rows already taken.  Rows are Python ints, x bits in the low 64 * W and z bits above, masked to the live
+    qubits.  Returns the rows and how many of them needed a redraw."""
+    W = 1 if num_qubits <= 64 else 2
+    gamma = 0x9E3779B97F4A7C15
+    masks = [(1 << min(64, max(0, num_qubits - 64 * w))) - 1 for w in range(W)]
+    rows = min(2 * num_qubits, 31)
+    pivots: dict[int, int] = {}  # leading bit -> reduced row
+
+    def reduce(v: int) -> int:
+        while v:
+            top = v.bit_length() - 1
+            if top not in pivots:
+                return v
+            v ^= pivots[top]
+        return 0
+
+    out, redrawn = [], 0
+    for r in range(rows):
+        for attempt in range(4097):
+            key = 0
+            for w in range(W):
+                for half in (0, 1):
+                    pos = (attempt << 32) | (r << 16) | (w << 1) | half
+                    word = py_mix64((seed + gamma * (pos + 1)) & MASK64) & masks[w]
+                    key |= word << (64 * (half * W + w))
+            red = reduce(key)
+            if red or attempt == 4096:
+                break
+        if red:
+            pivots[red.bit_length() - 1] = red
+        redrawn += attempt > 0
+        out.append(key)
+    return out, W, redrawn
+
+
+@pytest.mark.parametrize("seed", [DEFAULT_SEED, 0x1, 0xC0FFEE, 0xDEADBEEF12345678])
+def test_hash_rows_with_rank_completion_redraws(seed):
+    """The oracle's rows equal the independent restatement for every qubit count 1..32 (where draws are redrawn
+    until independent) and at 36 and 64 qubits (the one-word configurations), and the redraw branch is taken."""
+    total_redrawn = 0
+    for n in list(range(1, 33)) + [36, 64]:
+        want, W, redrawn = independent_hash_rows(n, seed)
+        total_redrawn += redrawn
+        rx, rz = po.hash_rows(n, seed, W)
+        got = []
+        for i in range(len(rx)):
+            key = 0
+            for w in range(W):
+                key |= int(rx[i, w]) << (64 * w)
+                key |= int(rz[i, w]) << (64 * (W + w))
+            got.append(key)
+        assert got == want, (n, seed)
+    assert total_redrawn > 0
+
+
 def test_rows_do_not_depend_on_the_width():
     # pauli_sum/hash/tests.rs:252-278
     for n in (1, 5, 64):
@@ -216,7 +274,7 @@ def test_untruncated_propagation_matches_brute_force(seed, direction):
     gates = []
     circ = C.Circuit(n)
     for _ in range(12):
-        k = int(rng.integers(1, 3))
+        k = int(rng.integers(1, 5))  # weight 3 and 4 generators prepare as non-local rotations
         qs = [int(q) for q in rng.choice(n, size=k, replace=False)]
         ps = "".join(rng.choice(list("XYZ"), size=k))
         theta = float(rng.uniform(-math.pi, math.pi))
@@ -297,7 +355,9 @@ def test_bucketed_layer_equals_the_naive_oracle(num_qubits, dtype, theta):
     W = x.shape[1]
     s = po.OracleSum(num_qubits, x, z, c)
     x0, z0, c0, _ = s.export()
-    for paulis, qubits in (("ZZ", [3, 7]), ("X", [5]), ("Y", [0]), ("XY", [num_qubits - 1, 1])):
+    wide = [0, 4, 11, 17, 19] if num_qubits < 64 else [62, 63, 64, 65, 99]
+    for paulis, qubits in (("ZZ", [3, 7]), ("X", [5]), ("Y", [0]), ("XY", [num_qubits - 1, 1]),
+                           ("XZY", wide[:3]), ("ZXXZY", wide)):
         ch = C.Circuit(num_qubits).pauli_rotation(paulis, qubits, theta).channels()
         pol = po.coeff_policy(eps=1e-3, max_weight=12)
         nx, nz, nc = po.naive_layer(W, x0, z0, c0, ch, False, pol)

@@ -141,6 +141,28 @@ def gpu_measure(x, z, c, channels, steps: int, warmup: int):
     }
 
 
+def stream_probe(bytes_per_array: int = 1 << 30, reps: int = 5) -> dict:
+    """The library's STREAM passes (pp_stream_probe): the HBM ceiling measured in this run, beside the 8 TB/s spec."""
+    import ctypes
+
+    from paulistrings_amd import _native
+
+    ctx = _native.Context.get(0)
+    out = (ctypes.c_double * 4)()
+    _native.check(_native.lib().pp_stream_probe(ctx.handle, bytes_per_array, reps, out), ctx.handle)
+    return {"read": out[0], "write": out[1], "copy": out[2], "triad": out[3]}
+
+
+def summarize(r: dict, dtype_name: str, steps: int) -> dict:
+    """One timed measurement as value, ms per step and the join's algorithmic rate."""
+    bytes_per_row = 2 * 2 * 8 + (8 if dtype_name == "f64" else 4)
+    kernel_s = r["kernel_ms"] / 1e3
+    achieved = (r["rows_in"] + r["rows_out"]) * bytes_per_row / kernel_s / 1e9 if kernel_s > 0 else 0.0
+    return {"dtype": dtype_name, "value": r["rows_out"] / r["seconds"], "unit": "terms/s",
+            "ms_per_step": r["seconds"] / steps * 1e3, "kernel_avg_ms": r["kernel_ms"] / max(1, r["launches"]),
+            "achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_row": bytes_per_row}
+
+
 def cpu_model() -> str:
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -557,6 +579,14 @@ def main() -> int:
     x, z, c = saturated_keys(1 << args.log2n, SEED, args.dtype)
     log(f"{len(c)} distinct keys; warmup {args.warmup}, timed {args.steps} steps of {len(channels)} layers")
     r = gpu_measure(x, z, c, channels, args.steps, args.warmup)
+    stream = stream_probe()
+    log("STREAM probe (GB/s): " + ", ".join(f"{k} {v:.0f}" for k, v in stream.items()))
+    other = None
+    if args.dtype == "f64":  # the same keys with the coefficients rounded to float32: the layer in float32
+        log(f"float32 coefficients: timed {args.steps} steps of {len(channels)} layers")
+        other = summarize(gpu_measure(x, z, c.astype(np.float32), channels, args.steps, args.warmup), "f32",
+                          args.steps)
+        log(f"float32: {other['value']:.3e} merged terms/s, join kernel {other['achieved']:.0f} GB/s")
     bytes_per_row = 2 * 2 * 8 + (8 if args.dtype == "f64" else 4)
     alg_bytes = (r["rows_in"] + r["rows_out"]) * bytes_per_row
     kernel_s = r["kernel_ms"] / 1e3
@@ -572,7 +602,7 @@ def main() -> int:
         log("cpu baseline on one thread")
         cpu1 = cpu_baseline(channels, args.dtype, args.cpu_log2n - 2, args.cpu_seconds / 3, threads=1)
 
-    traffic = None
+    traffic = None  # from the committed rocprofv3 --pmc summary of this workload (counters need their own run)
     if args.log2n == 24:
         prof = ROOT / "profiles" / f"rotation_join_saturated_layer_2e24_{args.dtype}_traffic.md"
         if prof.exists():
@@ -622,6 +652,11 @@ def main() -> int:
             "speedup_vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
             "cpu_baseline_one_thread": cpu1,
             "host": {"nproc": os.cpu_count(), "cpu_model": cpu_model()},
+            "traffic_source": "profiles/rotation_join_saturated_layer_2e24_" + args.dtype + "_traffic.md: rocprofv3 "
+                              "--pmc FETCH_SIZE and WRITE_SIZE of this workload, scaled by a calibrated copy",
+            "stream_probe_gbs": stream,
+            "achieved_over_stream_copy": achieved / stream["copy"] if stream.get("copy") else None,
+            "f32": other,
         },
     }
     print(json.dumps(out), flush=True)

@@ -128,7 +128,9 @@ pp_status pp_propagate(pp_sum* sum, const pp_channel* channels, uint64_t n_chann
                        const pp_policy* policy, const pp_options* options, uint64_t* layer_len);
 
 /* One rotation layer on the device's current bucketing, with no rebucket and no layer pass: the
- * kernel the benchmark times (DevicePartition::apply_layer, engine/gpu/partition.rs:315-374). */
+ * kernel the benchmark times.  It is PauliRotation::prepare_for_layer (channel/rotation.rs:94-123) and
+ * DevicePartition::apply_layer (engine/gpu/partition.rs:315-374) in one call, for a host that holds angles rather
+ * than prepared tables; pp_prepare_channel + pp_apply_prepared below are the two halves. */
 pp_status pp_apply_rotation(pp_sum* sum, const pp_channel* channel, int adjoint, const pp_policy* policy);
 
 /* The same layer out of place: dst receives the layer applied to src, which is left as it was (dst's previous
@@ -152,6 +154,79 @@ pp_status pp_truncate_approx_topn(pp_sum* sum, uint64_t n);
 pp_status pp_expectation_product(pp_sum* sum, const uint64_t* axis_x, const uint64_t* axis_z,
                                  const uint64_t* negative, double* out);
 
+/* ---- the layer seam: PartitionBackend / PartitionStorage (engine/backend.rs:26-108) ---------------- */
+
+/* What a host that keeps its own run_layers (engine/driver.rs:319-592), Channel and TruncationPolicy calls per
+ * layer, in place of pp_propagate.  The host prepares each channel (Channel::prepare_for_layer), decides the
+ * reference's bucket count and the runs (Runs::take, engine/driver.rs:202-258), and calls the layer pass after
+ * every run; the device keeps its own bucketing underneath. */
+
+/* DeltaEntry (channel/prepared.rs:35-46): one key delta of a local table.  Pattern s packs the support qubits'
+ * bits, bit 2j the x bit of support qubit j and bit 2j+1 its z bit (prepared.rs:141-151); amplitude[s] takes
+ * pattern s to s ^ local_delta, and an exact 0 emits nothing.  A float32 sum's host passes its float32 amplitudes
+ * widened to double; the device narrows them back exactly. */
+typedef struct {
+    uint32_t local_delta;
+    double amplitude[16];
+} pp_delta_entry;
+
+typedef enum {
+    PP_PREPARED_LOCAL = 0,   /* Prepared::Local(LocalPtm): support <= 2 qubits, tabulated (prepared.rs:79-86) */
+    PP_PREPARED_ROTATION = 1 /* Prepared::Rotation(PreparedRotation): a wider Pauli rotation (prepared.rs:303-317) */
+} pp_prepared_kind;
+
+/* Prepared (channel/prepared.rs:343-350).  LOCAL: k support qubits, ascending, and n_deltas entries ascending by
+ * local_delta (the identity first when present).  ROTATION: the generator and its cos and sin.  Bucket deltas
+ * are the device's own and not passed. */
+typedef struct {
+    int32_t kind;
+    uint32_t k;
+    uint32_t qubits[2];
+    uint32_t n_deltas;
+    pp_delta_entry deltas[16];
+    uint64_t gx[4];
+    uint64_t gz[4];
+    double cos;
+    double sin;
+} pp_prepared;
+
+/* PauliRotation::prepare_for_layer (channel/rotation.rs:94-123, Prepared::derive_local prepared.rs:401-426): the
+ * rotation's table on its support (weight <= 2) or its PreparedRotation, with sin and cos snapped at quarter turns
+ * (rotation.rs:70-78) and negated angle for the adjoint, amplitudes rounded to the coefficient type.  For a host
+ * that holds a PauliRotation; host-only (no device needed). */
+pp_status pp_prepare_channel(uint32_t num_qubits, uint32_t width, pp_coeff_kind kind, const pp_channel* channel,
+                             int adjoint, pp_prepared* out);
+
+/* PartitionBackend::apply_layer (engine/backend.rs:80-89; the device one engine/gpu/partition.rs:315-374): one
+ * prepared channel under the policy's per-term filter, no layer pass.  Supported: a table shaped like a rotation
+ * (RotationLayer::of, engine/bucketed/rotation_layer.rs:101-140) whose amplitudes are a rotation's (the pair join,
+ * its Clifford arm at quarter turns); a key-preserving table (one entry, local_delta 0: an in-place rescale by
+ * amplitude[s]); a wider rotation; any other local table of at most 16 entries (the general table layer).  The
+ * sum's length is exact on return. */
+pp_status pp_apply_prepared(pp_sum* sum, const pp_prepared* prepared, const pp_policy* policy);
+
+/* PartitionBackend::apply_run (engine/backend.rs:99-104): the layers of one run (Run::Rotations, at most 8
+ * rotation tables whose bucket deltas span at most 3 dimensions; or Run::Permutations, one-to-one steps) with the
+ * same keys, coefficients and per-layer lengths as layer by layer; a permutation run is walked in one pass.
+ * layer_len (optional, n entries) receives the length after each layer.  The caller keeps the run within the
+ * policy's run_length_limit (truncation/builtin.rs:147-149), as run_layers does. */
+typedef enum { PP_RUN_ROTATIONS = 0, PP_RUN_PERMUTATIONS = 1 } pp_run_kind;
+pp_status pp_apply_run(pp_sum* sum, pp_run_kind kind, const pp_prepared* layers, uint32_t n, const pp_policy* policy,
+                       uint64_t* layer_len);
+
+/* PartitionBackend::finalize_layer (engine/backend.rs:90; engine/gpu/partition.rs:439-447): the policy's layer
+ * pass (TopN or ApproxTopN; nothing for a per-term policy).  An exact TopN's retain is left to the next layer on
+ * the device, which reads its input through the selection; pp_settle (or any read of the sum) applies it. */
+pp_status pp_finalize_layer(pp_sum* sum, const pp_policy* policy);
+
+/* PartitionStorage::settle (engine/backend.rs:60-62; engine/gpu/partition.rs:270): apply what the last layer pass deferred. */
+pp_status pp_settle(pp_sum* sum);
+
+/* PartitionStorage::refine / coarsen_to (engine/backend.rs:35-37; engine/gpu/partition.rs:230-239): the reference's bucket count, which fixes the
+ * canonical order and TopN's choice among ties.  The device bucketing follows on its own. */
+pp_status pp_sum_refine(pp_sum* sum);
+pp_status pp_sum_coarsen_to(pp_sum* sum, int32_t bits);
+
 /* ---- measurement --------------------------------------------------------------------------- */
 
 /* While enabled, the rotation-join kernel of every layer is bracketed by hipEvents on the context's
@@ -163,6 +238,12 @@ pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms,
 /* The same for the learned pass's score kernel (pp_truncate_learned, pp_propagate_learned): its launches, their
  * total milliseconds and the terms they scored since profiling was enabled.  No counterpart in the reference. */
 pp_status pp_profile_read_score(pp_context* ctx, uint64_t* launches, double* total_ms, uint64_t* terms);
+
+/* An HBM ceiling probe beside the measurements (the reference's membench, crates/membench/kernels/membench.cu,
+ * crates/membench/src/gpu.rs:17-60): STREAM read, write, copy and triad passes over `bytes`-byte f64 arrays on the
+ * context's stream, each timed with hipEvents, best of `reps`.  gbs[0..4) receives read, write, copy and triad in
+ * GB/s, counting 1, 1, 2 and 3 arrays moved per pass.  It allocates its three arrays and frees them. */
+pp_status pp_stream_probe(pp_context* ctx, uint64_t bytes, uint32_t reps, double* gbs);
 
 /* ---- batches ------------------------------------------------------------------------------------ */
 

@@ -194,6 +194,43 @@ __device__ __forceinline__ bool survives(const KeepProgram& p, const Key<W>& k, 
     return stack & 1u;
 }
 
+// A key's pattern on a local table's two support slots (LocalPtm::support_bits, channel/prepared.rs:141-151): bit 2j
+// the x bit of slot j, bit 2j+1 its z bit; `mask` keeps the support's live bits (3 for one qubit, 0 for none).  The
+// slots' words and bits are uniform, so selecting a word costs a scalar branch or a v_cndmask.
+template <int W>
+__device__ __forceinline__ uint32_t pattern_of(const Key<W>& k, const uint32_t (&word)[2], const uint32_t (&bit)[2],
+                                               uint32_t mask) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        uint64_t xw = k.x[0], zw = k.z[0];
+        if (W > 1 && word[j] == 1) {
+            xw = k.x[W > 1 ? 1 : 0];
+            zw = k.z[W > 1 ? 1 : 0];
+        }
+        s |= (uint32_t)(((xw >> bit[j]) & 1ull) | (((zw >> bit[j]) & 1ull) << 1)) << (2 * j);
+    }
+    return s & mask;
+}
+
+// k ^ lift(d) (LocalPtm::lift, channel/prepared.rs:284-299) for a local delta d on the same two slots.
+template <int W>
+__device__ __forceinline__ void xor_local_delta(Key<W>& k, uint32_t d, const uint32_t (&word)[2],
+                                                const uint32_t (&bit)[2]) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const uint64_t mx = (uint64_t)((d >> (2 * j)) & 1u) << bit[j];
+        const uint64_t mz = (uint64_t)((d >> (2 * j + 1)) & 1u) << bit[j];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            if (word[j] == (uint32_t)w) {
+                k.x[w] ^= mx;
+                k.z[w] ^= mz;
+            }
+        }
+    }
+}
+
 // Row parity of a GF(2) hash row: bit r of H.v (pauli_sum/hash.rs:170-179).
 template <int W>
 __device__ __forceinline__ uint32_t row_parity(const Key<W>& k, const uint64_t* __restrict__ rows, int r) {

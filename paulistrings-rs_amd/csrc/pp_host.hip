@@ -20,6 +20,7 @@
 #include "../../include/pp_hip.h"
 #include "pp_kernels.h"
 #include "pp_learned.h"
+#include "pp_membench.h"
 #include "pp_topn.h"
 
 namespace {
@@ -479,7 +480,7 @@ pp_status ensure_tiles(pp_sum* s, uint32_t out_n) {
 
 // Slot `slot` of the sum's truncation scratch with room for `bytes` (grown to 1.5x the request when it must grow).
 template <typename P>
-pp_status scratch(pp_sum* s, ScratchSlot slot, size_t bytes, P** out) {
+pp_status slot_scratch(pp_sum* s, ScratchSlot slot, size_t bytes, P** out) {
     if (s->scratch_cap[slot] < bytes || !s->scratch[slot]) {
         if (s->scratch[slot]) (void)hipFree(s->scratch[slot]);
         s->scratch[slot] = nullptr;
@@ -620,37 +621,294 @@ pp_status settle_device_bits(pp_sum* s) {
 
 // ---- one prepared channel ------------------------------------------------------------------------------------------
 
+// The device's reading of a Prepared (channel/prepared.rs:343-350): which kernel applies it, and the run planner's
+// questions about it (max_fanout engine/bucketed.rs:628-641, PermutationStep::of permutation_run.rs:53-121,
+// RotationLayer::of rotation_layer.rs:101-140).
 struct Prepared {
+    pp_prepared table{};          // the channel's prepared form as the seam carries it
+    bool local = true;            // Prepared::Local (support <= 2) rather than Prepared::Rotation
+    bool rotation = false;        // the pair join applies it from (g, cos, sin): a rotation's own table or a wider one
+    uint64_t gx[4] = {}, gz[4] = {};
     double sin = 0, cos = 1;
     int weight = 0;
-    bool local = true;            // Prepared::Local (support <= 2) rather than Prepared::Rotation
-    bool key_preserving = false;  // only the identity delta survives
-    bool clifford = false;        // the identity residue is dropped (rotation_layer.rs:117-121)
+    bool key_preserving = false;  // one identity entry: an in-place rescale by amplitude[s]
+    bool clifford = false;        // rotation-shaped with |a0|^2 <= eps^2 |a1|^2 wherever a1 != 0 (drops the residue)
+    bool rotation_shaped = false; // RotationLayer::of: a rotation run may hold it
+    bool permutation = false;     // PermutationStep::of: one output per pattern, distinct targets
+    uint32_t fanout = 2;          // max_fanout
+    uint8_t step_delta[16] = {};  // the permutation step: per pattern its local delta and amplitude
+    double step_amp[16] = {};
 };
 
-Prepared prepare(const pp_sum* s, const pp_channel& ch, bool adjoint) {
+// Lift a local delta to full-width masks on the table's support (LocalPtm::lift, prepared.rs:284-299).
+void lift_local(const pp_prepared& t, uint32_t d, uint64_t* mx, uint64_t* mz) {
+    for (int w = 0; w < 4; ++w) mx[w] = mz[w] = 0;
+    for (uint32_t j = 0; j < t.k; ++j) {
+        const uint32_t q = t.qubits[j];
+        if ((d >> (2 * j)) & 1u) mx[q / 64] |= 1ull << (q % 64);
+        if ((d >> (2 * j + 1)) & 1u) mz[q / 64] |= 1ull << (q % 64);
+    }
+}
+
+// The phase exponent of P_s . g for local patterns s and g on the same support (pauli_string.rs:73-93), mod 4.
+uint32_t local_phase(uint32_t s, uint32_t g) {
+    uint32_t d = 0;
+    for (int j = 0; j < 2; ++j) {
+        const uint32_t a = (s >> (2 * j)) & 1u, b = (s >> (2 * j + 1)) & 1u;
+        const uint32_t c = (g >> (2 * j)) & 1u, e = (g >> (2 * j + 1)) & 1u;
+        d += 2u * (b & c) + (a & b) + (c & e) - ((a ^ c) & (b ^ e));
+    }
+    return d & 3u;
+}
+
+uint32_t local_anticommutes(uint32_t s, uint32_t g) {
+    uint32_t f = 0;
+    for (int j = 0; j < 2; ++j) f += (((s >> (2 * j)) & 1u) & ((g >> (2 * j + 1)) & 1u)) ^
+                                     (((s >> (2 * j + 1)) & 1u) & ((g >> (2 * j)) & 1u));
+    return f & 1u;
+}
+
+// The amplitude an f32 sum's table holds: the f64 value rounded to float, widened back.
+double as_coeff(const pp_sum* s, double v) { return s->kind == PP_COEFF_F64 ? v : (double)(float)v; }
+
+// PauliRotation::prepare_for_layer (channel/rotation.rs:94-123): weight <= 2 tabulates the rotation on its support
+// as Prepared::derive_local would probe it (prepared.rs:401-426, push_deltas :257-281): pattern s commuting with g
+// stays with amplitude 1; an anticommuting one stays with cos and moves to s ^ g with i sin (s g)'s real sign
+// (rotation.rs:43-67); entries whose amplitudes are all zero are not made.  Wider generators keep cos and sin.
+pp_prepared prepare_channel_table(const pp_sum* s, const pp_channel& ch, bool adjoint) {
+    pp_prepared t{};
+    double sv, cv;
+    snapped_sin_cos(adjoint ? -ch.theta : ch.theta, &sv, &cv);
+    int weight = 0;
+    for (uint32_t w = 0; w < s->W; ++w) weight += __builtin_popcountll(ch.gx[w] | ch.gz[w]);
+    if (weight > 2) {
+        t.kind = PP_PREPARED_ROTATION;
+        for (uint32_t w = 0; w < 4; ++w) {
+            t.gx[w] = w < s->W ? ch.gx[w] : 0;
+            t.gz[w] = w < s->W ? ch.gz[w] : 0;
+        }
+        t.cos = cv;
+        t.sin = sv;
+        return t;
+    }
+    t.kind = PP_PREPARED_LOCAL;
+    for (uint32_t w = 0; w < s->W; ++w) {
+        uint64_t live = ch.gx[w] | ch.gz[w];
+        while (live) {
+            t.qubits[t.k++] = 64 * w + (uint32_t)__builtin_ctzll(live);
+            live &= live - 1;
+        }
+    }
+    uint32_t g = 0;  // the generator's own pattern
+    for (uint32_t j = 0; j < t.k; ++j) {
+        const uint32_t q = t.qubits[j];
+        g |= (uint32_t)((ch.gx[q / 64] >> (q % 64)) & 1u) << (2 * j);
+        g |= (uint32_t)((ch.gz[q / 64] >> (q % 64)) & 1u) << (2 * j + 1);
+    }
+    const uint32_t dim = 1u << (2 * t.k);
+    double ident[16] = {}, move[16] = {};
+    bool moves = false;
+    for (uint32_t p = 0; p < dim; ++p) {
+        if (!local_anticommutes(p, g)) {
+            ident[p] = 1.0;
+            continue;
+        }
+        ident[p] = cv;
+        const uint32_t total = (1u + local_phase(p, g)) & 3u;
+        move[p] = (total & 2u) ? -sv : sv;
+        moves = moves || move[p] != 0.0;
+    }
+    bool any_ident = false;
+    for (uint32_t p = 0; p < dim; ++p) any_ident = any_ident || ident[p] != 0.0;
+    if (any_ident) {
+        pp_delta_entry& e = t.deltas[t.n_deltas++];
+        e.local_delta = 0;
+        for (uint32_t p = 0; p < dim; ++p) e.amplitude[p] = as_coeff(s, ident[p]);
+    }
+    if (moves && g != 0) {
+        pp_delta_entry& e = t.deltas[t.n_deltas++];
+        e.local_delta = g;
+        for (uint32_t p = 0; p < dim; ++p) e.amplitude[p] = as_coeff(s, move[p]);
+    }
+    return t;
+}
+
+// Check a prepared channel against the sum and read it (Prepared::Local's invariants, prepared.rs:79-86).
+pp_status classify(pp_sum* s, const pp_prepared& t, Prepared* out) {
     Prepared p;
-    snapped_sin_cos(adjoint ? -ch.theta : ch.theta, &p.sin, &p.cos);
-    for (uint32_t w = 0; w < s->W; ++w) p.weight += __builtin_popcountll(ch.gx[w] | ch.gz[w]);
-    p.local = p.weight <= 2;
-    p.key_preserving = p.weight == 0 || p.sin == 0.0;
-    const double a0 = s->kind == PP_COEFF_F64 ? p.cos : (double)(float)p.cos;
-    const double a1 = s->kind == PP_COEFF_F64 ? p.sin : (double)(float)p.sin;
+    p.table = t;
     const double eps = 2.220446049250313e-16;
-    p.clifford = !p.key_preserving && a0 * a0 <= eps * eps * (a1 * a1);
+    if (t.kind == PP_PREPARED_ROTATION) {
+        p.local = false;
+        p.rotation = true;
+        for (uint32_t w = 0; w < 4; ++w) {
+            const uint64_t live = w < s->W ? live_mask(s->num_qubits, w) : 0;
+            if ((t.gx[w] | t.gz[w]) & ~live)
+                return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "prepared rotation acts outside the register");
+            p.gx[w] = t.gx[w];
+            p.gz[w] = t.gz[w];
+            p.weight += __builtin_popcountll(t.gx[w] | t.gz[w]);
+        }
+        p.cos = t.cos;
+        p.sin = t.sin;
+        // a wide rotation is never key-preserving for the planner, even at sin = 0: the join then drops every zero
+        // partner and rescales the anticommuting rows, as the reference's Prepared::Rotation layer does
+        const double a0 = as_coeff(s, t.cos), a1 = as_coeff(s, t.sin);
+        p.clifford = a1 != 0.0 && a0 * a0 <= eps * eps * (a1 * a1);
+        p.fanout = 2;
+        *out = p;
+        return PP_OK;
+    }
+    if (t.kind != PP_PREPARED_LOCAL || t.k > 2 || t.n_deltas > 16)
+        return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "prepared table: kind, support or entry count out of range");
+    for (uint32_t j = 0; j < t.k; ++j)
+        if (t.qubits[j] >= s->num_qubits || (j > 0 && t.qubits[j] <= t.qubits[j - 1]))
+            return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "prepared table: support qubits not ascending in range");
+    const uint32_t dim = 1u << (2 * t.k);
+    for (uint32_t e = 0; e < t.n_deltas; ++e) {
+        if (t.deltas[e].local_delta >= dim || (e > 0 && t.deltas[e].local_delta <= t.deltas[e - 1].local_delta))
+            return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "prepared table: local deltas not ascending within the support");
+        for (uint32_t q = 0; q < 16; ++q)
+            if (t.deltas[e].amplitude[q] != as_coeff(s, t.deltas[e].amplitude[q]) ||
+                (q >= dim && t.deltas[e].amplitude[q] != 0.0))
+                return fail(s->ctx, PP_ERR_INVALID_ARGUMENT,
+                            "prepared table: an amplitude outside the support's patterns or the coefficient type");
+    }
+    p.weight = (int)t.k;
+    // max_fanout (engine/bucketed.rs:628-641)
+    uint32_t fan = 0;
+    for (uint32_t q = 0; q < dim; ++q) {
+        uint32_t f = 0;
+        for (uint32_t e = 0; e < t.n_deltas; ++e) f += t.deltas[e].amplitude[q] != 0.0;
+        fan = std::max(fan, f);
+    }
+    p.fanout = std::max(fan, 1u);
+    p.key_preserving = t.n_deltas == 0 || (t.n_deltas == 1 && t.deltas[0].local_delta == 0);
+    // RotationLayer::of: an identity entry and one generator entry; where a1 = 0 the identity amplitude is 1, and the
+    // patterns a1 reaches are closed under ^ g
+    if (t.n_deltas == 2 && t.deltas[0].local_delta == 0) {
+        const pp_delta_entry &e0 = t.deltas[0], &e1 = t.deltas[1];
+        const uint32_t g = e1.local_delta;
+        bool shaped = true, clifford = true;
+        for (uint32_t q = 0; q < dim && shaped; ++q) {
+            const double a0 = e0.amplitude[q], a1 = e1.amplitude[q];
+            if (a1 == 0.0) {
+                shaped = a0 == 1.0;
+            } else {
+                shaped = e1.amplitude[q ^ g] != 0.0;
+                clifford = clifford && a0 * a0 <= eps * eps * (a1 * a1);
+            }
+        }
+        p.rotation_shaped = shaped;
+        p.clifford = shaped && clifford;
+        if (shaped && p.clifford) p.fanout = 1;
+        // a rotation's own table: a0 = cos on every pattern that anticommutes with g, a1 = +-sin by (s g)'s phase,
+        // and a1 = 0, a0 = 1 on the others; the join then needs only (g, cos, sin)
+        if (shaped) {
+            int first = -1;
+            for (uint32_t q = 0; q < dim; ++q)
+                if (local_anticommutes(q, g)) {
+                    first = (int)q;
+                    break;
+                }
+            bool exact = first >= 0;
+            double cv = 1.0, sv = 0.0;
+            if (exact) {
+                cv = e0.amplitude[first];
+                sv = (((1u + local_phase((uint32_t)first, g)) & 3u) & 2u) ? -e1.amplitude[first] : e1.amplitude[first];
+            }
+            for (uint32_t q = 0; q < dim && exact; ++q) {
+                if (!local_anticommutes(q, g)) {
+                    exact = e0.amplitude[q] == 1.0 && e1.amplitude[q] == 0.0;
+                } else {
+                    const double a1 = (((1u + local_phase(q, g)) & 3u) & 2u) ? -sv : sv;
+                    exact = e0.amplitude[q] == cv && e1.amplitude[q] == a1 && std::signbit(e1.amplitude[q]) == std::signbit(a1);
+                }
+            }
+            if (exact && sv != 0.0) {
+                p.rotation = true;
+                p.cos = cv;
+                p.sin = sv;
+                lift_local(t, g, p.gx, p.gz);
+            }
+        }
+    }
+    // PermutationStep::of: per pattern one emitting entry (for a Clifford rotation table the generator entry where
+    // it emits, else the identity), reaching distinct patterns
+    if (t.k > 0) {
+        bool one_to_one = true;
+        bool reached[16] = {};
+        for (uint32_t q = 0; q < dim && one_to_one; ++q) {
+            int emit = -1;
+            if (p.rotation_shaped) {
+                if (!p.clifford) {
+                    one_to_one = false;
+                    break;
+                }
+                emit = t.deltas[1].amplitude[q] != 0.0 ? 1 : 0;
+            } else {
+                for (uint32_t e = 0; e < t.n_deltas; ++e)
+                    if (t.deltas[e].amplitude[q] != 0.0) {
+                        if (emit >= 0) one_to_one = false;
+                        emit = (int)e;
+                    }
+            }
+            if (!one_to_one) break;
+            if (emit < 0) continue;  // the pattern is dropped (an exact zero)
+            const uint32_t to = q ^ t.deltas[emit].local_delta;
+            if (reached[to]) one_to_one = false;
+            reached[to] = true;
+            p.step_delta[q] = (uint8_t)t.deltas[emit].local_delta;
+            p.step_amp[q] = t.deltas[emit].amplitude[q];
+        }
+        p.permutation = one_to_one;
+    }
+    if (p.key_preserving) {
+        for (uint32_t q = 0; q < 16; ++q) {
+            p.step_delta[q] = 0;
+            p.step_amp[q] = t.n_deltas ? t.deltas[0].amplitude[q] : 0.0;
+        }
+    }
+    *out = p;
+    return PP_OK;
+}
+
+// The prepared form of a PauliRotation channel, read as the device applies it.
+Prepared prepare(pp_sum* s, const pp_channel& ch, bool adjoint) {
+    Prepared p;
+    const pp_status st = classify(s, prepare_channel_table(s, ch, adjoint), &p);
+    (void)st;  // a rotation's own table always classifies
     return p;
 }
 
-size_t fanout(const Prepared& p) {  // max_fanout, engine/bucketed.rs:628-641
-    if (!p.local) return 2;
-    return (p.key_preserving || p.clifford) ? 1 : 2;
+// The device form of a one-output-per-pattern table under the current device bucketing.
+TableStep table_step(const pp_sum* s, const Prepared& p) {
+    TableStep st{};
+    const pp_prepared& t = p.table;
+    const uint32_t q0 = t.k ? t.qubits[0] : 0, q1 = t.k ? t.qubits[t.k - 1] : 0;
+    st.word[0] = q0 / 64;
+    st.bit[0] = q0 % 64;
+    st.word[1] = q1 / 64;
+    st.bit[1] = q1 % 64;
+    st.mask = (1u << (2 * t.k)) - 1u;
+    const uint32_t dim = 1u << (2 * t.k);
+    for (uint32_t q = 0; q < 16; ++q) {
+        const uint32_t d = q < dim ? p.step_delta[q] : 0u;
+        st.delta[q] = (uint8_t)d;
+        st.amp[q] = q < dim ? p.step_amp[q] : 1.0;
+        uint64_t mx[4], mz[4];
+        lift_local(t, d, mx, mz);
+        st.bucket[q] = bucket_of_host(s, mx, mz, s->device_bits);
+        if (d == 0 && st.amp[q] == 1.0) st.inert |= 1u << q;
+    }
+    return st;
 }
 
-bool is_permutation_step(const Prepared& p) {  // PermutationStep::of, permutation_run.rs:53-121
-    return p.local && p.weight > 0 && (p.key_preserving || p.clifford);
-}
+size_t fanout(const Prepared& p) { return p.fanout; }  // max_fanout, engine/bucketed.rs:628-641
 
-bool is_rotation_table(const Prepared& p) { return p.local && !p.key_preserving; }  // RotationLayer::of
+bool is_permutation_step(const Prepared& p) { return p.local && p.permutation; }  // PermutationStep::of
+
+bool is_rotation_table(const Prepared& p) { return p.local && p.rotation_shaped; }  // RotationLayer::of
 
 int span_dimension(std::vector<uint32_t> v) {
     int rank = 0;
@@ -704,16 +962,20 @@ pp_status refresh(pp_sum* s);
 
 // The rotation join from `src`'s current columns into `dst`'s column set `out` (dst may be src, with out its
 // spare set).  Asynchronous: the counters land in dst->d_stats.
-pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, const Prepared& p,
-                      const KeepProgram& keep) {
+// The output rows a join from src needs in dst's column set, and the buckets.
+uint64_t join_output_rows(const pp_sum* src) {
+    const uint64_t rows_held = src->pending_topn ? src->phys_len : src->len;
+    return 2 * rows_held + (uint64_t)kSegAlign * (1ull << src->device_bits);
+}
+
+pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, const KeepProgram& keep) {
     pp_context* ctx = src->ctx;
     pp_status st;
     const int in = src->cur;
     const uint32_t B = 1u << src->device_bits;
-    const uint64_t rows_held = src->pending_topn ? src->phys_len : src->len;
-    if ((st = ensure_columns(dst, out, 2 * rows_held + (uint64_t)kSegAlign * B)) != PP_OK) return st;
+    if ((st = ensure_columns(dst, out, join_output_rows(src))) != PP_OK) return st;
     if ((st = ensure_buckets(dst, out, B)) != PP_OK) return st;
-    const uint32_t delta = bucket_of_host(src, ch.gx, ch.gz, src->device_bits);
+    const uint32_t delta = bucket_of_host(src, p.gx, p.gz, src->device_bits);
     // the scan also clears dst's counters
     if ((st = offsets(dst, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start, true)) != PP_OK) return st;
     // lcap bounds the largest pair; the first launch stages scap anticommuting rows per pair, and the pairs with
@@ -740,6 +1002,8 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
     }
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     const bool timed = ctx->profiling && !ctx->capturing;
+    unsigned long long* const stats = dst->d_stats;
+    uint32_t* const redo = dst->d_redo;
     // Up to kJoinCountPairs pairs, the join's workgroups add the totals themselves (one same-address atomic each)
     // and the length-reduce launch is skipped; measurement keeps the reduce, which also feeds the profile counter.
     constexpr uint32_t kJoinCountPairs = 512;
@@ -772,18 +1036,18 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         a.num_cosets = cosets;
         uint32_t gphase = 0;
         for (uint32_t w = 0; w < W; ++w) {
-            a.gx[w] = ch.gx[w];
-            a.gz[w] = ch.gz[w];
-            gphase += (uint32_t)__builtin_popcountll(ch.gx[w] & ch.gz[w]);
+            a.gx[w] = p.gx[w];
+            a.gz[w] = p.gz[w];
+            gphase += (uint32_t)__builtin_popcountll(p.gx[w] & p.gz[w]);
         }
         a.gphase = gphase & 3u;
         a.cos_t = (T)p.cos;
         a.sin_t = (T)p.sin;
         a.clifford = p.clifford ? 1 : 0;
         a.keep = keep;
-        a.stats = dst->d_stats;
-        a.redo = dst->d_redo;
-        a.redo_count = dst->d_stats + 4;
+        a.stats = stats;
+        a.redo = redo;
+        a.redo_count = stats + 4;
         a.count_totals = count_in_join ? 1u : 0u;
         a.filter = src->pending_topn ? src->d_topn : nullptr;
         a.filter_flags = src->pending_topn ? src->pending_flags : nullptr;
@@ -802,18 +1066,21 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const pp_channel& ch, c
         ctx->prof_rows_in += src->len;
     }
     if (!count_in_join)
-        PP_HIP(ctx, launch_reduce_lens(ctx->stream, dst->bk[out].len, B, dst->d_stats, timed ? ctx->d_prof : nullptr));
+        PP_HIP(ctx, launch_reduce_lens(ctx->stream, dst->bk[out].len, B, stats, timed ? ctx->d_prof : nullptr));
     return PP_OK;
 }
 
-pp_status device_layer(pp_sum* s, const pp_channel& ch, const Prepared& p, const KeepProgram& keep) {
+pp_status device_layer(pp_sum* s, const Prepared& p, const KeepProgram& keep) {
     pp_context* ctx = s->ctx;
     pp_status st;
+    if (!p.key_preserving && !p.rotation)
+        return fail(ctx, PP_ERR_UNSUPPORTED, "a local table of this shape has no device layer yet");
     if (p.key_preserving && (st = materialize_topn(s)) != PP_OK) return st;
     if ((st = settle_device_bits(s)) != PP_OK) return st;
     const uint32_t B = 1u << s->device_bits;
     if (p.key_preserving) {
         if ((st = clear_stats(s)) != PP_OK) return st;
+        const TableStep step = table_step(s, p);
         hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
             constexpr int W = decltype(Wc)::value;
             using T = decltype(Tv);
@@ -824,11 +1091,7 @@ pp_status device_layer(pp_sum* s, const pp_channel& ch, const Prepared& p, const
             a.cap = s->col[s->cur].cap;
             a.start = s->bk[s->cur].start;
             a.len = s->bk[s->cur].len;
-            for (uint32_t w = 0; w < W; ++w) {
-                a.gx[w] = ch.gx[w];
-                a.gz[w] = ch.gz[w];
-            }
-            a.a0 = (T)p.cos;
+            a.step = step;
             a.keep = keep;
             a.stats = s->d_stats;
             return launch_rescale<W, T>(ctx->stream, a, B);
@@ -837,7 +1100,7 @@ pp_status device_layer(pp_sum* s, const pp_channel& ch, const Prepared& p, const
         return count_layout(s, s->cur, B);
     }
     const int out = 1 - s->cur;
-    if ((st = launch_join(s, s, out, ch, p, keep)) != PP_OK) return st;
+    if ((st = launch_join(s, s, out, p, keep)) != PP_OK) return st;
     s->pending_topn = false;  // the join read its input through the selection
     if ((st = read_stats(s)) != PP_OK) return st;
     if (s->h_stats[2]) return fail(ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
@@ -858,7 +1121,7 @@ bool fused_permutation_runs() {
 // the steps in registers, in place, counting each output bucket and each step's drops; an offset scan lays out the
 // destinations; k_perm_scatter moves the rows into the spare columns.  len_after[j] receives the length after step
 // j, as the per-layer path would report it.
-pp_status permutation_run_device(pp_sum* s, const PermStep* steps, uint32_t n_steps, const KeepProgram& keep,
+pp_status permutation_run_device(pp_sum* s, const Prepared* steps, uint32_t n_steps, const KeepProgram& keep,
                                  uint64_t* len_after) {
     pp_context* ctx = s->ctx;
     pp_status st;
@@ -866,24 +1129,21 @@ pp_status permutation_run_device(pp_sum* s, const PermStep* steps, uint32_t n_st
     const uint32_t B = 1u << s->device_bits;
     uint32_t *d_dest = nullptr, *d_counts = nullptr;
     unsigned long long* d_deaths = nullptr;
-    PermStep* d_steps = nullptr;
-    if ((st = scratch(s, kScrRunCounts, (size_t)B * 4, &d_counts)) != PP_OK ||
-        (st = scratch(s, kScrDeaths, (size_t)kMaxPermSteps * 8, &d_deaths)) != PP_OK ||
-        (st = scratch(s, kScrSteps, (size_t)kMaxPermSteps * sizeof(PermStep), &d_steps)) != PP_OK)
+    TableStep* d_steps = nullptr;
+    if ((st = slot_scratch(s, kScrRunCounts, (size_t)B * 4, &d_counts)) != PP_OK ||
+        (st = slot_scratch(s, kScrDeaths, (size_t)kMaxPermSteps * 8, &d_deaths)) != PP_OK ||
+        (st = slot_scratch(s, kScrSteps, (size_t)kMaxPermSteps * sizeof(TableStep), &d_steps)) != PP_OK)
         return st;
     const uint64_t len_in = s->len;
     std::vector<unsigned long long> deaths(n_steps, 0);
     for (uint32_t first = 0; first < n_steps; first += kMaxPermSteps) {
         const uint32_t m = std::min(kMaxPermSteps, n_steps - first);
         const int in = s->cur, out = 1 - s->cur;
-        if ((st = scratch(s, kScrDest, s->col[in].cap * 4, &d_dest)) != PP_OK) return st;
-        // every step's device bucket delta under the current bucketing
-        std::vector<PermStep> chunk(steps + first, steps + first + m);
-        for (PermStep& p : chunk) {
-            uint64_t gx[4] = {p.gx[0], p.gx[1], 0, 0}, gz[4] = {p.gz[0], p.gz[1], 0, 0};
-            p.delta = bucket_of_host(s, gx, gz, s->device_bits);
-        }
-        PP_HIP(ctx, hipMemcpyAsync(d_steps, chunk.data(), m * sizeof(PermStep), hipMemcpyHostToDevice, ctx->stream));
+        if ((st = slot_scratch(s, kScrDest, s->col[in].cap * 4, &d_dest)) != PP_OK) return st;
+        // every step's table with its device bucket deltas under the current bucketing
+        std::vector<TableStep> chunk(m);
+        for (uint32_t j = 0; j < m; ++j) chunk[j] = table_step(s, steps[first + j]);
+        PP_HIP(ctx, hipMemcpyAsync(d_steps, chunk.data(), m * sizeof(TableStep), hipMemcpyHostToDevice, ctx->stream));
         PP_HIP(ctx, hipMemsetAsync(d_counts, 0, (size_t)B * 4, ctx->stream));
         PP_HIP(ctx, hipMemsetAsync(d_deaths, 0, (size_t)m * 8, ctx->stream));
         hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
@@ -944,6 +1204,18 @@ pp_status permutation_run_device(pp_sum* s, const PermStep* steps, uint32_t n_st
     return PP_OK;
 }
 
+// The counts of an asynchronous out-of-place layer, without applying a selection left pending for the next layer.
+pp_status refresh_counts(pp_sum* s) {
+    if (!s->stale) return PP_OK;
+    pp_status st = read_stats(s);
+    if (st != PP_OK) return st;
+    s->stale = false;
+    if (s->h_stats[2]) return fail(s->ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
+    s->len = s->h_stats[0];
+    s->max_bucket = (uint32_t)s->h_stats[1];
+    return PP_OK;
+}
+
 // Bring len and max_bucket up to date after an asynchronous out-of-place layer (and apply a selection an
 // interrupted propagation left pending).
 pp_status refresh(pp_sum* s) {
@@ -958,9 +1230,7 @@ pp_status refresh(pp_sum* s) {
     return PP_OK;
 }
 
-pp_status apply_one(pp_sum* s, const pp_channel& ch, const Prepared& p, const KeepProgram& keep) {
-    return device_layer(s, ch, p, keep);
-}
+pp_status apply_one(pp_sum* s, const Prepared& p, const KeepProgram& keep) { return device_layer(s, p, keep); }
 
 pp_status topn_device(pp_sum* s, uint64_t n, const float* score = nullptr,
                       bool defer = false);  // pp_topn_impl.inc
@@ -1037,31 +1307,15 @@ struct RunPlanner {
     pp_status apply_run(pp_sum* s, uint64_t i, uint64_t end, const KeepProgram& keep, uint64_t* len_after) const {
         pp_status st;
         bool walk = end > i + 1 && s->len > 0 && fused_permutation_runs();
-        std::vector<PermStep> steps;
+        std::vector<Prepared> steps;
         for (uint64_t j = i; walk && j < end; ++j) {
-            const pp_channel& ch = channel_at(j);
-            const Prepared p = prepare(s, ch, adjoint);
-            if (!is_permutation_step(p)) {
-                walk = false;
-                break;
-            }
-            PermStep q{};
-            uint32_t gphase = 0;
-            for (uint32_t w = 0; w < s->W; ++w) {
-                q.gx[w] = ch.gx[w];
-                q.gz[w] = ch.gz[w];
-                q.supp |= ((ch.gx[w] | ch.gz[w]) != 0 ? 1u : 0u) << w;
-                gphase += (uint32_t)__builtin_popcountll(ch.gx[w] & ch.gz[w]);
-            }
-            q.gphase = gphase & 3u;
-            q.move = p.key_preserving ? 0u : 1u;
-            q.amp = p.key_preserving ? p.cos : p.sin;
-            steps.push_back(q);
+            steps.push_back(prepare(s, channel_at(j), adjoint));
+            walk = is_permutation_step(steps.back());
         }
         if (walk) return permutation_run_device(s, steps.data(), (uint32_t)steps.size(), keep, len_after);
         for (uint64_t j = i; j < end; ++j) {
             const Prepared q = prepare(s, channel_at(j), adjoint);
-            if ((st = apply_one(s, channel_at(j), q, keep)) != PP_OK) return st;
+            if ((st = apply_one(s, q, keep)) != PP_OK) return st;
             if (len_after) len_after[j - i] = s->len;
         }
         return PP_OK;
@@ -1342,7 +1596,7 @@ pp_status pp_sum_clone(const pp_sum* src_c, pp_sum** out) {
 pp_status pp_sum_len(pp_sum* s, uint64_t* len) {
     if (s) PP_NOT_CAPTURING(s->ctx, "pp_sum_len");
     if (!s || !len) return PP_ERR_INVALID_ARGUMENT;
-    pp_status st = refresh(s);
+    pp_status st = refresh_counts(s);  // exact also while a selection waits for its retain: len is the kept count
     if (st != PP_OK) return st;
     *len = s->len;
     return PP_OK;
@@ -1466,7 +1720,7 @@ pp_status pp_apply_rotation(pp_sum* s, const pp_channel* ch, int adjoint, const 
     pp_status st = refresh(s);
     if (st != PP_OK) return st;
     const Prepared p = prepare(s, *ch, adjoint != 0);
-    return apply_one(s, *ch, p, lower_policy(policy));
+    return apply_one(s, p, lower_policy(policy));
 }
 
 pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* ch, int adjoint, const pp_policy* policy) {
@@ -1494,7 +1748,7 @@ pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* ch, i
     dst->device_bits = src->device_bits;
     dst->reference_bits = src->reference_bits;
     const int out = dst->cur;
-    if ((st = launch_join(src, dst, out, *ch, p, lower_policy(policy))) != PP_OK) return st;
+    if ((st = launch_join(src, dst, out, p, lower_policy(policy))) != PP_OK) return st;
     dst->stale = true;
     if (ctx->capturing) {
         const int in = src->cur;
@@ -1514,6 +1768,105 @@ pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* ch, i
         r.src_max_bucket = src->max_bucket;
         ctx->capture_log.push_back(r);
     }
+    return PP_OK;
+}
+
+// ---- the layer seam ------------------------------------------------------------------------------------------------
+
+pp_status pp_prepare_channel(uint32_t num_qubits, uint32_t width, pp_coeff_kind kind, const pp_channel* ch,
+                             int adjoint, pp_prepared* out) {
+    if (!ch || !out || num_qubits == 0 || num_qubits > 128 || (width != 1 && width != 2) ||
+        (width == 1) != (num_qubits <= 64) || (kind != PP_COEFF_F64 && kind != PP_COEFF_F32))
+        return PP_ERR_INVALID_ARGUMENT;
+    for (uint32_t w = 0; w < 4; ++w) {
+        const uint64_t live = w < width ? live_mask(num_qubits, w) : 0;
+        if ((ch->gx[w] | ch->gz[w]) & ~live) return PP_ERR_INVALID_ARGUMENT;
+    }
+    pp_sum shape;  // the preparation reads only the register's width and the coefficient type
+    shape.num_qubits = num_qubits;
+    shape.W = width;
+    shape.kind = kind;
+    *out = prepare_channel_table(&shape, *ch, adjoint != 0);
+    return PP_OK;
+}
+
+pp_status pp_apply_prepared(pp_sum* s, const pp_prepared* prepared, const pp_policy* policy) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_apply_prepared");
+    if (!s || !prepared) return PP_ERR_INVALID_ARGUMENT;
+    if (!valid_policy(policy)) return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
+    pp_status st;
+    if ((st = refresh_counts(s)) != PP_OK) return st;
+    Prepared p;
+    if ((st = classify(s, *prepared, &p)) != PP_OK) return st;
+    return apply_one(s, p, lower_policy(policy));
+}
+
+pp_status pp_apply_run(pp_sum* s, pp_run_kind kind, const pp_prepared* layers, uint32_t n, const pp_policy* policy,
+                       uint64_t* layer_len) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_apply_run");
+    if (!s || (n && !layers)) return PP_ERR_INVALID_ARGUMENT;
+    if (!valid_policy(policy)) return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
+    if (kind != PP_RUN_ROTATIONS && kind != PP_RUN_PERMUTATIONS)
+        return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "run kind");
+    if (kind == PP_RUN_ROTATIONS && n > kRunLayers)
+        return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "a rotation run holds at most 8 layers");
+    pp_status st;
+    if ((st = refresh_counts(s)) != PP_OK) return st;
+    std::vector<Prepared> preps(n);
+    for (uint32_t j = 0; j < n; ++j) {
+        if ((st = classify(s, layers[j], &preps[j])) != PP_OK) return st;
+        const bool fits = kind == PP_RUN_PERMUTATIONS ? is_permutation_step(preps[j]) : is_rotation_table(preps[j]);
+        if (!fits)
+            return fail(s->ctx, PP_ERR_INVALID_ARGUMENT,
+                        "run layer " + std::to_string(j) +
+                            (kind == PP_RUN_PERMUTATIONS ? " is not a one-to-one step" : " is not rotation-shaped"));
+    }
+    const KeepProgram keep = lower_policy(policy);
+    if (kind == PP_RUN_PERMUTATIONS && n > 0 && s->len > 0)
+        return permutation_run_device(s, preps.data(), n, keep, layer_len);
+    for (uint32_t j = 0; j < n; ++j) {
+        if (s->len == 0) {  // nothing left to transform: the remaining layers keep it empty
+            if (layer_len) layer_len[j] = 0;
+            continue;
+        }
+        if ((st = apply_one(s, preps[j], keep)) != PP_OK) return st;
+        if (layer_len) layer_len[j] = s->len;
+    }
+    return PP_OK;
+}
+
+pp_status pp_finalize_layer(pp_sum* s, const pp_policy* policy) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_finalize_layer");
+    if (!s) return PP_ERR_INVALID_ARGUMENT;
+    if (!valid_policy(policy)) return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
+    if (!policy || !policy->has_topn) return PP_OK;
+    pp_status st;
+    if ((st = refresh_counts(s)) != PP_OK) return st;
+    if (policy->has_topn == 2) return pp_truncate_approx_topn(s, policy->topn);
+    if (s->pending_topn && (st = materialize_topn(s)) != PP_OK) return st;  // two passes in a row
+    if (s->len <= policy->topn) return PP_OK;
+    return topn_device(s, policy->topn, nullptr, true);
+}
+
+pp_status pp_settle(pp_sum* s) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_settle");
+    if (!s) return PP_ERR_INVALID_ARGUMENT;
+    return refresh(s);
+}
+
+pp_status pp_sum_refine(pp_sum* s) {
+    if (!s) return PP_ERR_INVALID_ARGUMENT;
+    if (s->reference_bits >= s->max_bits)
+        return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "refine: the hash has no further row");
+    s->reference_bits += 1;
+    return PP_OK;
+}
+
+pp_status pp_sum_coarsen_to(pp_sum* s, int32_t bits) {
+    if (!s) return PP_ERR_INVALID_ARGUMENT;
+    if (bits < 0 || bits > s->reference_bits)
+        return fail(s->ctx, PP_ERR_INVALID_ARGUMENT, "coarsen_to: bits above the current count");
+    s->reference_bits = bits;
     return PP_OK;
 }
 
@@ -1662,6 +2015,53 @@ pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms,
     return PP_OK;
 }
 
+pp_status pp_stream_probe(pp_context* ctx, uint64_t bytes, uint32_t reps, double* gbs) {
+    if (!ctx || !gbs || reps == 0 || bytes < 4096) return PP_ERR_INVALID_ARGUMENT;
+    PP_NOT_CAPTURING(ctx, "pp_stream_probe");
+    const uint64_t n2 = bytes / 16;
+    double* arr[3] = {};
+    double* partial = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    pp_status st = PP_OK;
+    auto cleanup = [&]() {
+        for (double* p : arr)
+            if (p) (void)hipFree(p);
+        if (partial) (void)hipFree(partial);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    for (double*& p : arr)
+        if ((st = dev_alloc(ctx, (void**)&p, n2 * 16)) != PP_OK) {
+            cleanup();
+            return st;
+        }
+    if ((st = dev_alloc(ctx, (void**)&partial, stream_blocks(n2) * 8)) != PP_OK ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        cleanup();
+        return st != PP_OK ? st : fail(ctx, PP_ERR_DRIVER, "pp_stream_probe: events");
+    }
+    const double moved[4] = {1, 1, 2, 3};
+    for (int pass = 0; pass < 4; ++pass) {
+        double best = 0.0;
+        for (uint32_t r = 0; r <= reps; ++r) {  // the first pass warms up
+            hipError_t e = hipEventRecord(e0, ctx->stream);
+            if (e == hipSuccess) e = launch_stream_pass(ctx->stream, pass, arr[0], arr[1], arr[2], partial, n2);
+            if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                cleanup();
+                return fail(ctx, PP_ERR_DRIVER, std::string("pp_stream_probe: ") + hipGetErrorString(e));
+            }
+            if (r > 0 && ms > 0.f) best = std::max(best, moved[pass] * (double)(n2 * 16) / (ms * 1e-3) / 1e9);
+        }
+        gbs[pass] = best;
+    }
+    cleanup();
+    return PP_OK;
+}
+
 pp_status pp_profile_read_score(pp_context* ctx, uint64_t* launches, double* total_ms, uint64_t* terms) {
     PP_NOT_CAPTURING(ctx, "pp_profile_read_score");
     if (!ctx) return PP_ERR_INVALID_ARGUMENT;
@@ -1766,8 +2166,8 @@ pp_status batch_step(pp_batch* b, const std::vector<uint32_t>& idx, const std::v
         const uint64_t rows_held = s->pending_topn ? s->phys_len : s->len;
         if ((st = ensure_columns(s, out, 2 * rows_held + (uint64_t)kSegAlign * B)) != PP_OK) return st;
         if ((st = ensure_buckets(s, out, B)) != PP_OK) return st;
-        const pp_channel& ch = chans[k];
-        const uint32_t delta = bucket_of_host(s, ch.gx, ch.gz, s->device_bits);
+        const Prepared& p = preps[k];
+        const uint32_t delta = bucket_of_host(s, p.gx, p.gz, s->device_bits);
         uint32_t lcap = delta ? 2u * s->max_bucket : s->max_bucket;
         lcap = std::max<uint32_t>(64, (lcap + 63) & ~63u);
         if (lcap > kMaxPairRows) lcap = kMaxPairRows;
@@ -1793,9 +2193,9 @@ pp_status batch_step(pp_batch* b, const std::vector<uint32_t>& idx, const std::v
         a.num_cosets = delta ? B / 2 : B;
         uint32_t gphase = 0;
         for (uint32_t w = 0; w < W; ++w) {
-            a.gx[w] = ch.gx[w];
-            a.gz[w] = ch.gz[w];
-            gphase += (uint32_t)__builtin_popcountll(ch.gx[w] & ch.gz[w]);
+            a.gx[w] = p.gx[w];
+            a.gz[w] = p.gz[w];
+            gphase += (uint32_t)__builtin_popcountll(p.gx[w] & p.gz[w]);
         }
         a.gphase = gphase & 3u;
         a.cos_t = (T)preps[k].cos;
@@ -1876,7 +2276,7 @@ pp_status pp_batch_propagate(pp_batch* b, const pp_channel* channels, const uint
             if ((st = settle_device_bits(s)) != PP_OK) return st;
             const Prepared p = prepare(s, ch, plan.adjoint);
             if (p.key_preserving) {  // in place, on its own
-                if ((st = device_layer(s, ch, p, keeps[i])) != PP_OK) return st;
+                if ((st = device_layer(s, p, keeps[i])) != PP_OK) return st;
                 if (++next[i] == run_end[i] && (st = plan.finish_run(s, next[i] < plan.n_channels)) != PP_OK)
                     return st;
                 continue;

@@ -51,6 +51,23 @@ struct JoinArgs {
     const float* filter_score;    // its ranking score per input position when it ranked by score, else null (|c|)
 };
 
+// A local table applied row by row, one output per row (a permutation step, PermutationStep::of,
+// engine/bucketed/permutation_run.rs:53-121, or a key-preserving rescale): the key's pattern on the table's support
+// (bit 2j the x bit of support slot j, bit 2j+1 its z bit, channel/prepared.rs:141-151; a one-qubit table repeats
+// its qubit in slot 1, masked off) selects the local delta it moves by and the amplitude it takes.
+struct TableStep {
+    double amp[16];       // exactly 0 drops the row
+    uint32_t bucket[16];  // the device bucket delta of lift(delta[s]) under the bucketing of the launch
+    uint8_t delta[16];
+    uint32_t word[2];
+    uint32_t bit[2];
+    uint32_t mask;    // the patterns' live bits: 15 with two support qubits, 3 with one, 0 with none
+    uint32_t inert;   // bit s: pattern s keeps its key and its coefficient exactly
+};
+constexpr uint32_t kMaxPermSteps = 256;  // steps per walk launch (a longer run takes several)
+constexpr uint32_t kWalkChunk = 32;      // steps a walk workgroup holds in LDS at once
+constexpr uint32_t kDeadRow = 0xFFFFFFFFu;
+
 template <int W, typename T>
 struct RescaleArgs {
     uint64_t* x;
@@ -59,25 +76,10 @@ struct RescaleArgs {
     uint64_t cap;
     const uint64_t* start;
     uint32_t* len;
-    uint64_t gx[W];
-    uint64_t gz[W];
-    T a0;
+    TableStep step;  // key-preserving: every delta 0
     KeepProgram keep;
     unsigned long long* stats;
 };
-
-// One step of a permutation run (a Clifford rotation or a key-preserving one) as the device walks it.
-struct PermStep {
-    uint64_t gx[2];
-    uint64_t gz[2];
-    double amp;       // move: the sine the moved rows take (partner_amplitude_on); else the rows' cosine factor
-    uint32_t supp;    // words of g's support
-    uint32_t gphase;  // |gx & gz| mod 4
-    uint32_t delta;   // device bucket delta of g
-    uint32_t move;    // 1: anticommuting rows move to k ^ g (the Clifford arm); 0: they are rescaled in place
-};
-constexpr uint32_t kMaxPermSteps = 256;  // steps per walk launch (a longer run takes several)
-constexpr uint32_t kDeadRow = 0xFFFFFFFFu;
 
 template <int W, typename T>
 struct PermWalkArgs {
@@ -87,7 +89,7 @@ struct PermWalkArgs {
     uint64_t cap;
     const uint64_t* start;
     const uint32_t* len;
-    const PermStep* steps;
+    const TableStep* steps;
     uint32_t num_steps;
     KeepProgram keep;
     uint32_t* dest;                 // per column position: the row's output bucket, or kDeadRow

@@ -385,8 +385,9 @@ __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const 
 // REDO: the launch over the queued pairs (a compile-time switch, so that the first launch's pair index needs no
 // load: a lookup there, merged with the prefetch's path, would make the compiler wait for the prefetch).
 // The join of work items t, t + G, t + G + G, ... below count (see k_rotation_join); lds and scan_scratch are the
-// workgroup's.
-template <int W, typename T, bool REDO, int BLOCK>
+// workgroup's.  FILTER: the input may carry a pending TopN selection (a compile-time switch: the arguments it reads
+// stay out of the scalar registers of the common launch, which has none).
+template <int W, typename T, bool REDO, int BLOCK, bool FILTER>
 __device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, uint32_t G, uint32_t count,
                                            unsigned char* lds, uint32_t* scan_scratch) {
     const uint32_t scap = a.scap;  // staged rows this launch holds
@@ -413,7 +414,9 @@ __device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, 
     const uint32_t supp = support_words<W>(g);
     const uint32_t fold_g = key_fold<W>(g);
     TopnState fs{};
-    if (a.filter) fs = *a.filter;
+    if constexpr (FILTER) {
+        if (a.filter) fs = *a.filter;
+    }
 
     uint32_t p = pair_at(t);
     PairMeta m = read_meta(a, p, load_meta(a, p));
@@ -453,7 +456,7 @@ __device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, 
             const uint32_t r = base + threadIdx.x;
             bool valid = r < n;
             const uint32_t j = r >= m.n0 ? 1u : 0u;
-            if (valid && a.filter) {  // uniform test: a pending TopN retain
+            if (FILTER && valid && a.filter) {  // uniform test: a pending TopN retain
                 const uint64_t i = j ? m.s1 + (r - m.n0) : m.s0 + r;
                 const uint64_t u = a.filter_score ? score_bits(a.filter_score[i]) : magnitude_bits<T>(cb.c);
                 valid = topn_keeps(fs, u, !a.filter_score && is_nan_bits(u), a.filter_flags, i);
@@ -547,12 +550,12 @@ __device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, 
     }
 }
 
-template <int W, typename T, bool REDO, int BLOCK>
+template <int W, typename T, bool REDO, int BLOCK, bool FILTER>
 __global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t scan_scratch[4 * (BLOCK / 64)];
     const uint32_t count = REDO ? (uint32_t)*a.redo_count : a.num_cosets;
-    join_items<W, T, REDO, BLOCK>(a, blockIdx.x, gridDim.x, count, lds, scan_scratch);
+    join_items<W, T, REDO, BLOCK, FILTER>(a, blockIdx.x, gridDim.x, count, lds, scan_scratch);
 }
 
 // Many sums' layers in one launch: workgroup w joins pair w - pair_start[i] of item i, where item i is the one
@@ -571,7 +574,7 @@ __global__ void __launch_bounds__(BLOCK) k_rotation_join_batch(const JoinArgs<W,
     }
     const JoinArgs<W, T>& a = items[lo];
     const uint32_t t = blockIdx.x - pair_start[lo];
-    join_items<W, T, false, BLOCK>(a, t, a.num_cosets, a.num_cosets, lds, scan_scratch);
+    join_items<W, T, false, BLOCK, true>(a, t, a.num_cosets, a.num_cosets, lds, scan_scratch);
 }
 
 // The batch's bucket offsets and counters, one workgroup per item: the output segments of a join (as
@@ -637,22 +640,20 @@ __global__ void __launch_bounds__(kBlock) k_reduce_batch(const JoinArgs<W, T>* _
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// A key-preserving layer (a rotation whose sine snapped to zero): rescale in place with the zero drop and
-// keep_term (engine/bucketed.rs:646-662).  One workgroup per bucket compacts its segment in order.
+// A key-preserving layer (a table whose only entry is the identity delta, e.g. a rotation whose sine snapped to zero):
+// rescale in place by the pattern's amplitude, with the zero drop and keep_term (engine/bucketed.rs:646-662).  One
+// workgroup per bucket compacts its segment in order.
 // ---------------------------------------------------------------------------------------------------------------
 
 template <int W, typename T>
 __global__ void __launch_bounds__(kBlock) k_rescale(const RescaleArgs<W, T> a) {
     __shared__ uint32_t scan_scratch[kWaves];
+    __shared__ T amp[16];
+    if (threadIdx.x < 16) amp[threadIdx.x] = (T)a.step.amp[threadIdx.x];
+    __syncthreads();
     const uint32_t b = blockIdx.x;
     const uint32_t n = a.len[b];
     const uint64_t s = a.start[b];
-    Key<W> g;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-        g.x[w] = a.gx[w];
-        g.z[w] = a.gz[w];
-    }
     uint32_t cur = 0;
     for (uint32_t base = 0; base < n; base += kBlock) {
         const uint32_t r = base + threadIdx.x;
@@ -662,7 +663,7 @@ __global__ void __launch_bounds__(kBlock) k_rescale(const RescaleArgs<W, T> a) {
         if (r < n) {
             k = load_key<W>(a.x, a.z, a.cap, s + r);
             const T c = a.c[s + r];
-            v = anticommutes<W>(k, g) ? c * a.a0 : c;
+            v = c * amp[pattern_of<W>(k, a.step.word, a.step.bit, a.step.mask)];
             keep = survives<W, T>(a.keep, k, v);
         }
         bool f[1] = {keep};
@@ -683,61 +684,71 @@ __global__ void __launch_bounds__(kBlock) k_rescale(const RescaleArgs<W, T> a) {
 
 // ---------------------------------------------------------------------------------------------------------------
 // A permutation run (engine/bucketed/permutation_run.rs:1-4, :216-293, :297-...): steps that map distinct keys to
-// distinct keys, so no two rows ever meet and each row walks the whole run alone, in registers.  Per step a row
-// that anticommutes with g is either moved to k ^ g with c * a1(k) (a Clifford rotation, the join's arm that drops
-// the identity residue: the coefficient the join gives that key is 0 + c * a1(k), the same value) or rescaled to
-// c * cos (a key-preserving rotation, k_rescale's product); a row that commutes is unchanged.  A row is dropped
-// where the per-layer path drops it: after a step that changed it, when it no longer survives, and when it never
-// changed or first changes after step 0, where the first layer would have refused it unchanged.  The walk counts
-// every row's output bucket and every step's drops; k_perm_scatter then moves the rows into their buckets.
+// distinct keys, so no two rows ever meet and each row walks the whole run alone, in registers.  Per step the row's
+// pattern on the step's support selects one output: the key moved by that pattern's local delta with the
+// coefficient times its amplitude (a Clifford rotation's arm that drops the identity residue, a key-preserving
+// rescale, or any other one-to-one local table); an inert pattern leaves the row as it is.  A row is dropped where
+// the per-layer path drops it: after a step that changed it, when it no longer survives, and when it never changed
+// or first changes after step 0, where the first layer would have refused it unchanged.  The walk counts every
+// row's output bucket and every step's drops; k_perm_scatter then moves the rows into their buckets.  The steps
+// pass through LDS kWalkChunk at a time, each row's state staying in registers across the chunks.
 // ---------------------------------------------------------------------------------------------------------------
 
 template <int W, typename T>
 __global__ void __launch_bounds__(kBlock) k_perm_walk(const PermWalkArgs<W, T> a,
-                                                      const PermStep* __restrict__ steps) {
+                                                      const TableStep* __restrict__ steps) {
+    __shared__ TableStep lsteps[kWalkChunk];
     __shared__ uint32_t died[kMaxPermSteps];
     for (uint32_t j = threadIdx.x; j < a.num_steps; j += kBlock) died[j] = 0;
-    __syncthreads();
     const uint32_t b = blockIdx.x;
     const uint32_t n = a.len[b];
     const uint64_t s0 = a.start[b];
-    for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
+    constexpr uint32_t kStepWords = sizeof(TableStep) / 8;
+    static_assert(sizeof(TableStep) % 8 == 0, "TableStep is copied as 8-byte words");
+    for (uint32_t base = 0; base < n; base += kBlock) {  // uniform: every thread reaches the barriers below
+        const uint32_t r = base + threadIdx.x;
+        const bool live = r < n;
         const uint64_t i = s0 + r;
-        Key<W> k = load_key<W>(a.x, a.z, a.cap, i);
-        T c = a.c[i];
+        Key<W> k;
+        T c = T(0);
+        if (live) {
+            k = load_key<W>(a.x, a.z, a.cap, i);
+            c = a.c[i];
+        }
         uint32_t dest = b;
         bool tested = false;  // whether the row, as it now is, has been asked survives
-        int dead_at = -1;
-        // Every lane runs every step (a dropped row idles): the step index stays uniform, so the steps are read
-        // through the scalar cache rather than by a vector load per lane and step.
-        for (uint32_t j = 0; j < a.num_steps; ++j) {
-            const PermStep& st = steps[j];
-            Key<W> g;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                g.x[w] = st.gx[w];
-                g.z[w] = st.gz[w];
+        int dead_at = live ? -1 : -2;
+        for (uint32_t j0 = 0; j0 < a.num_steps; j0 += kWalkChunk) {
+            const uint32_t m = min(kWalkChunk, a.num_steps - j0);
+            __syncthreads();  // the previous chunk is no longer read
+            {
+                const uint64_t* src = reinterpret_cast<const uint64_t*>(steps + j0);
+                uint64_t* dst = reinterpret_cast<uint64_t*>(lsteps);
+                for (uint32_t w = threadIdx.x; w < m * kStepWords; w += kBlock) dst[w] = src[w];
             }
-            if (dead_at >= 0 || !anticommutes_on<W>(k, g, st.supp)) continue;
-            if (!tested && j > 0 && !survives<W, T>(a.keep, k, c)) {
-                dead_at = 0;
-                continue;
+            __syncthreads();
+            for (uint32_t jj = 0; jj < m; ++jj) {
+                if (dead_at != -1) continue;
+                const TableStep& st = lsteps[jj];
+                const uint32_t p = pattern_of<W>(k, st.word, st.bit, st.mask);
+                if ((st.inert >> p) & 1u) continue;
+                const uint32_t j = j0 + jj;
+                if (!tested && j > 0 && !survives<W, T>(a.keep, k, c)) {
+                    dead_at = 0;
+                    continue;
+                }
+                c = c * (T)st.amp[p];
+                xor_local_delta<W>(k, st.delta[p], st.word, st.bit);
+                dest ^= st.bucket[p];
+                tested = true;
+                if (!survives<W, T>(a.keep, k, c)) dead_at = (int)j;
             }
-            if (st.move) {
-                c = c * partner_amplitude_on<W, T>(k, g, st.supp, st.gphase, (T)st.amp);
-                k = key_xor<W>(k, g);
-                dest ^= st.delta;
-            } else {
-                c = c * (T)st.amp;
-            }
-            tested = true;
-            if (!survives<W, T>(a.keep, k, c)) dead_at = (int)j;
         }
-        if (dead_at < 0 && !tested && !survives<W, T>(a.keep, k, c)) dead_at = 0;
+        if (dead_at == -1 && !tested && !survives<W, T>(a.keep, k, c)) dead_at = 0;
         if (dead_at >= 0) {
             atomicAdd(&died[dead_at], 1u);
             a.dest[i] = kDeadRow;
-        } else {
+        } else if (dead_at == -1) {
             store_key<W>(a.x, a.z, a.cap, i, k);
             store_coeff<T>(a.c, i, c);
             a.dest[i] = dest;
@@ -956,17 +967,22 @@ hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_
     if (a.num_cosets == 0) return hipSuccess;
     const size_t lds = join_lds_bytes<W, T>(a.scap, a.tcap_bits);
     static int cus = 0;
-    static int resident[2][2][17] = {};  // by block, launch kind and scap / 64 (scap <= 1024, a multiple of 64)
+    static int resident[2][2][2][17] = {};  // by block, launch kind, filter and scap / 64 (scap <= 1024)
     if (cus == 0) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
     }
-    const bool wide = block == 512;
-    const auto kernel = a.redo_mode ? (wide ? k_rotation_join<W, T, true, 512> : k_rotation_join<W, T, true, 256>)
-                                    : (wide ? k_rotation_join<W, T, false, 512> : k_rotation_join<W, T, false, 256>);
-    int& per = resident[wide ? 1 : 0][a.redo_mode ? 1 : 0][std::min<uint32_t>(a.scap >> 6, 16u)];
+    const bool wide = block == 512, filter = a.filter != nullptr;
+    using Kernel = void (*)(const JoinArgs<W, T>);
+    static constexpr Kernel kernels[2][2][2] = {
+        {{k_rotation_join<W, T, false, 256, false>, k_rotation_join<W, T, false, 256, true>},
+         {k_rotation_join<W, T, false, 512, false>, k_rotation_join<W, T, false, 512, true>}},
+        {{k_rotation_join<W, T, true, 256, false>, k_rotation_join<W, T, true, 256, true>},
+         {k_rotation_join<W, T, true, 512, false>, k_rotation_join<W, T, true, 512, true>}}};
+    const Kernel kernel = kernels[a.redo_mode ? 1 : 0][wide ? 1 : 0][filter ? 1 : 0];
+    int& per = resident[wide ? 1 : 0][a.redo_mode ? 1 : 0][filter ? 1 : 0][std::min<uint32_t>(a.scap >> 6, 16u)];
     if (per == 0 &&
         (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, lds) != hipSuccess || per <= 0))
         per = 1;

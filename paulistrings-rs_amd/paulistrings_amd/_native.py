@@ -57,6 +57,32 @@ class Channel(ctypes.Structure):
     _fields_ = [("gx", ctypes.c_uint64 * 4), ("gz", ctypes.c_uint64 * 4), ("theta", ctypes.c_double)]
 
 
+class DeltaEntry(ctypes.Structure):
+    """pp_delta_entry: one key delta of a prepared local table (DeltaEntry, channel/prepared.rs:35-46)."""
+
+    _fields_ = [("local_delta", ctypes.c_uint32), ("amplitude", ctypes.c_double * 16)]
+
+
+PREPARED_LOCAL, PREPARED_ROTATION = 0, 1
+RUN_ROTATIONS, RUN_PERMUTATIONS = 0, 1
+
+
+class Prepared(ctypes.Structure):
+    """pp_prepared: a channel prepared for one layer (Prepared, channel/prepared.rs:343-350)."""
+
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("k", ctypes.c_uint32),
+        ("qubits", ctypes.c_uint32 * 2),
+        ("n_deltas", ctypes.c_uint32),
+        ("deltas", DeltaEntry * 16),
+        ("gx", ctypes.c_uint64 * 4),
+        ("gz", ctypes.c_uint64 * 4),
+        ("cos", ctypes.c_double),
+        ("sin", ctypes.c_double),
+    ]
+
+
 class NativeError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
@@ -115,6 +141,15 @@ def lib() -> ctypes.CDLL:
     L.pp_profile_read.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64),
                                   ctypes.POINTER(u64)]
     L.pp_profile_read_score.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)]
+    L.pp_prepare_channel.argtypes = [u32, u32, ctypes.c_int, ctypes.POINTER(Channel), ctypes.c_int,
+                                     ctypes.POINTER(Prepared)]
+    L.pp_apply_prepared.argtypes = [vp, ctypes.POINTER(Prepared), ctypes.POINTER(Policy)]
+    L.pp_apply_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Prepared), u32, ctypes.POINTER(Policy), vp]
+    L.pp_finalize_layer.argtypes = [vp, ctypes.POINTER(Policy)]
+    L.pp_settle.argtypes = [vp]
+    L.pp_sum_refine.argtypes = [vp]
+    L.pp_sum_coarsen_to.argtypes = [vp, i32]
+    L.pp_stream_probe.argtypes = [vp, u64, u32, ctypes.POINTER(ctypes.c_double)]
     L.pp_hash_rows.argtypes = [u32, u64, u32, vp, vp]
     L.pp_hash_rows.restype = i32
     for name in ("pp_context_create", "pp_sync", "pp_context_stream", "pp_sum_upload", "pp_sum_clone",
@@ -123,7 +158,9 @@ def lib() -> ctypes.CDLL:
                  "pp_graph_capture_begin", "pp_graph_capture_end", "pp_graph_launch", "pp_graph_free",
                  "pp_get_tuning", "pp_set_tuning", "pp_autotune", "pp_batch_create", "pp_batch_free",
                  "pp_batch_propagate",
-                 "pp_expectation_product", "pp_profile_enable", "pp_profile_read", "pp_profile_read_score"):
+                 "pp_expectation_product", "pp_profile_enable", "pp_profile_read", "pp_profile_read_score",
+                 "pp_prepare_channel", "pp_apply_prepared", "pp_apply_run", "pp_finalize_layer", "pp_settle",
+                 "pp_sum_refine", "pp_sum_coarsen_to", "pp_stream_probe"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -99,3 +99,70 @@ def test_circuit_encoding():
     with pytest.raises(ValueError):
         Circuit(4).rx(0.1, 4)
     assert math.isclose(circuits.tfim_square_step(2, 2, 0.05, 1.0, 1.0).channels()[-1]["theta"], 0.1)
+
+
+def prepared_entries(p, k):
+    """(local_delta, amplitudes) of a pp_prepared local table."""
+    return [(int(p.deltas[e].local_delta), [float(v) for v in p.deltas[e].amplitude]) for e in range(p.n_deltas)]
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_prepared_rotation_tables_equal_the_conjugation_ptm(kind, adjoint):
+    """pp_prepare_channel's table of exp(-i theta P / 2) on every one- and two-qubit generator: the entries and their
+    zero pattern equal the transfer matrix U P_s U^dag computed from 2^k x 2^k matrices (adjoint: theta -> -theta),
+    the values agree to 1e-14, and the values themselves are exactly the snapped glibc sin and cos (the oracle's,
+    rounded to float32 for kind 1) with the matrices' signs.  Host-only: no device."""
+    import ptm
+
+    L = _native.lib()
+    rng = np.random.default_rng(77 + kind + 2 * adjoint)
+    nq = 70
+    cases = []
+    for k in (1, 2):
+        for g in range(1, 1 << (2 * k)):
+            if k == 2 and (g & 3 == 0 or g >> 2 == 0):
+                continue  # weight 1: covered by k = 1
+            qs = sorted(int(q) for q in rng.choice(nq, size=k, replace=False))
+            for theta in (float(rng.uniform(-math.pi, math.pi)), math.pi / 2, -math.pi / 2, math.pi, 0.0):
+                cases.append((k, qs, g, theta))
+    for k, qs, g, theta in cases:
+        ch = _native.Channel()
+        for j, q in enumerate(qs):
+            ch.gx[q // 64] |= ((g >> (2 * j)) & 1) << (q % 64)
+            ch.gz[q // 64] |= ((g >> (2 * j + 1)) & 1) << (q % 64)
+        ch.theta = theta
+        p = _native.Prepared()
+        assert L.pp_prepare_channel(nq, 2, kind, ctypes.byref(ch), 1 if adjoint else 0, ctypes.byref(p)) == 0
+        assert p.kind == _native.PREPARED_LOCAL and p.k == k and list(p.qubits)[:k] == qs
+        th = -theta if adjoint else theta
+        gm = ptm.pauli_matrix(g, k)
+        u = math.cos(th / 2) * np.eye(1 << k) - 1j * math.sin(th / 2) * gm
+        want = ptm.entries_of(ptm.conjugation_ptm(u, k), k, clean=1e-14)
+        got = prepared_entries(p, k)
+        assert [d for d, _ in got] == [d for d, _ in want], (k, qs, g, theta)
+        s, c = po.sin_cos(th)
+        rnd = (lambda v: float(np.float32(v))) if kind == 1 else float
+        for (d, a), (_, b) in zip(got, want):
+            for pat in range(16):
+                assert (a[pat] == 0.0) == (b[pat] == 0.0), (k, g, theta, d, pat)
+                assert abs(a[pat] - b[pat]) < 1e-6 if kind == 1 else abs(a[pat] - b[pat]) < 1e-14
+                if a[pat] != 0.0 and d != 0:
+                    assert abs(a[pat]) == rnd(abs(s))  # the generator entry carries +-sin
+                elif a[pat] != 0.0 and a[pat] != 1.0:
+                    assert a[pat] == rnd(c)  # the identity entry carries cos where it acts
+
+
+def test_prepared_wide_rotation_keeps_cos_and_sin():
+    L = _native.lib()
+    ch = _native.Channel()
+    ch.gx[0] = 0b1011
+    ch.gz[1] = 1 << 3
+    ch.theta = 0.7
+    p = _native.Prepared()
+    assert L.pp_prepare_channel(100, 2, 0, ctypes.byref(ch), 1, ctypes.byref(p)) == 0
+    s, c = po.sin_cos(-0.7)
+    assert p.kind == _native.PREPARED_ROTATION and p.sin == s and p.cos == c
+    assert p.gx[0] == 0b1011 and p.gz[1] == 8
+    ch.gx[1] = 1 << 40  # outside a 100-qubit register's second word
+    assert L.pp_prepare_channel(100, 2, 0, ctypes.byref(ch), 0, ctypes.byref(p)) == 5

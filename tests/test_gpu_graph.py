@@ -103,8 +103,9 @@ def test_replay_is_refused_once_the_captured_buffers_changed():
 
     rng = np.random.default_rng(43)
     nq = 60
-    x, z = random_keys(rng, 1 << 14, nq, 0.1)
-    c = rng.standard_normal(1 << 14)
+    n = 12000  # its layer stays below the next device refinement (64 buckets of 256): the layer flips once
+    x, z = random_keys(rng, n, nq, 0.1)
+    c = rng.standard_normal(n)
     src = PauliSum(nq, x, z, c).to_device(0)
     dst = src.clone()
     ctx = src.ctx
@@ -118,7 +119,9 @@ def test_replay_is_refused_once_the_captured_buffers_changed():
     got = terms(dst)
     assert all(np.array_equal(a, b) for a, b in zip(got, want))
     assert _native.lib().pp_sum_free(dst._h) == 5  # PP_ERR_INVALID_ARGUMENT: the graph writes it
+    bits = dst.bucket_bits()[1]
     dst.apply_rotation(Circuit(nq).rx(0.3, 7).channels(), policy=policy)  # in place: the current set flips
+    assert dst.bucket_bits()[1] == bits  # no refinement in between (a refinement flips it back)
     with pytest.raises(Exception, match="changed since the capture"):
         cap.graph.launch()
     after = terms(dst)  # the in-place layer's result, untouched by the refused replay

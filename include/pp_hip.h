@@ -140,6 +140,16 @@ pp_status pp_apply_rotation(pp_sum* sum, const pp_channel* channel, int adjoint,
 pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* channel, int adjoint,
                                const pp_policy* policy);
 
+/* pp_apply_rotation_to with no device allocation, for callers that cannot allocate (tight serving loops, code that
+ * must not call hipMalloc); no counterpart in the reference.  The join's scratch (offset-scan tile sums, counters,
+ * redo queue) lives in a fixed per-GPU array of the library, for inputs of at most 2^18 device buckets (~2^26
+ * terms); dst's columns and buckets must already have room (one pp_apply_rotation_to of the same input sizes them)
+ * and src's device bucketing must be settled; otherwise it returns PP_ERR_INVALID_ARGUMENT / PP_ERR_UNSUPPORTED
+ * before queuing anything.  Calls on one GPU share that array: calls that overlap on one GPU (from two contexts'
+ * streams) race.  Not capturable into a graph. */
+pp_status pp_apply_rotation_to_noalloc(pp_sum* src, pp_sum* dst, const pp_channel* channel, int adjoint,
+                                       const pp_policy* policy);
+
 /* TopN(n) as a standalone pass (truncation/builtin.rs:73-129): ties at the n-th magnitude are kept
  * in the reference's canonical order. */
 pp_status pp_truncate_topn(pp_sum* sum, uint64_t n);

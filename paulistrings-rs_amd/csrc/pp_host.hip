@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -962,22 +963,38 @@ pp_status refresh(pp_sum* s);
 
 // The rotation join from `src`'s current columns into `dst`'s column set `out` (dst may be src, with out its
 // spare set).  Asynchronous: the counters land in dst->d_stats.
+// Device memory a join uses beyond its input and output: the offset scan's tile sums, the counters (kStatSlots) and the
+// queue of pairs for the redo launch.  By default dst's own; the no-allocation entry point passes a fixed buffer.
+struct JoinScratch {
+    uint64_t* tiles;
+    unsigned long long* stats;
+    uint32_t* redo;
+};
+
 // The output rows a join from src needs in dst's column set, and the buckets.
 uint64_t join_output_rows(const pp_sum* src) {
     const uint64_t rows_held = src->pending_topn ? src->phys_len : src->len;
     return 2 * rows_held + (uint64_t)kSegAlign * (1ull << src->device_bits);
 }
 
-pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, const KeepProgram& keep) {
+pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, const KeepProgram& keep,
+                      const JoinScratch* fixed = nullptr) {
     pp_context* ctx = src->ctx;
     pp_status st;
     const int in = src->cur;
     const uint32_t B = 1u << src->device_bits;
-    if ((st = ensure_columns(dst, out, join_output_rows(src))) != PP_OK) return st;
-    if ((st = ensure_buckets(dst, out, B)) != PP_OK) return st;
+    if (!fixed) {
+        if ((st = ensure_columns(dst, out, join_output_rows(src))) != PP_OK) return st;
+        if ((st = ensure_buckets(dst, out, B)) != PP_OK) return st;
+    }
     const uint32_t delta = bucket_of_host(src, p.gx, p.gz, src->device_bits);
     // the scan also clears dst's counters
-    if ((st = offsets(dst, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start, true)) != PP_OK) return st;
+    if (fixed) {
+        PP_HIP(ctx, launch_bucket_offsets(ctx->stream, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start,
+                                          fixed->stats + 3, fixed->tiles, fixed->stats, kStatSlots));
+    } else if ((st = offsets(dst, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start, true)) != PP_OK) {
+        return st;
+    }
     // lcap bounds the largest pair; the first launch stages scap anticommuting rows per pair, and the pairs with
     // more (rare: a generator anticommutes with a small fraction of a random bucket) go to a redo launch that
     // stages lcap.  LDS per workgroup, and so the resident workgroups per CU, follow scap.
@@ -993,7 +1010,7 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, cons
     const uint32_t cosets = delta ? B / 2 : B;
     // sized for B whatever this channel's cosets, so that no later layer on the same bucketing reallocates it (a
     // captured graph holds the pointer)
-    if (dst->redo_cap < B) {
+    if (!fixed && dst->redo_cap < B) {
         if (dst->d_redo) (void)hipFree(dst->d_redo);
         dst->d_redo = nullptr;
         dst->redo_cap = 0;
@@ -1001,9 +1018,9 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, cons
         dst->redo_cap = B;
     }
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    const bool timed = ctx->profiling && !ctx->capturing;
-    unsigned long long* const stats = dst->d_stats;
-    uint32_t* const redo = dst->d_redo;
+    const bool timed = ctx->profiling && !ctx->capturing && !fixed;
+    unsigned long long* const stats = fixed ? fixed->stats : dst->d_stats;
+    uint32_t* const redo = fixed ? fixed->redo : dst->d_redo;
     // Up to kJoinCountPairs pairs, the join's workgroups add the totals themselves (one same-address atomic each)
     // and the length-reduce launch is skipped; measurement keeps the reduce, which also feeds the profile counter.
     constexpr uint32_t kJoinCountPairs = 512;
@@ -1329,6 +1346,61 @@ struct RunPlanner {
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------------------------
+// The no-allocation join's scratch: one fixed array per GPU, for callers that cannot allocate device memory
+// (pp_apply_rotation_to_noalloc).  Every call on a GPU shares that GPU's copy, which keeps the last call's contents;
+// the counters and the redo count are cleared by each call's offset scan before the join reads them.  Calls that
+// overlap on one GPU (two contexts' streams) race on it.
+// ---------------------------------------------------------------------------------------------------------------
+
+constexpr uint32_t kNoAllocMaxBuckets = 1u << 18;          // device buckets of the largest input (~2^26 terms)
+constexpr size_t kNoAllocStatsOffset = 0;                  // kStatSlots counters
+constexpr size_t kNoAllocTilesOffset = 256;                // offsets_tiles(kNoAllocMaxBuckets) tile sums
+constexpr size_t kNoAllocRedoOffset = 2048;                // kNoAllocMaxBuckets queued pairs
+constexpr size_t kNoAllocScratchBytes = kNoAllocRedoOffset + (size_t)kNoAllocMaxBuckets * 4;
+static_assert(kStatSlots * 8 <= kNoAllocTilesOffset, "counters overlap the tile sums");
+static_assert(kNoAllocTilesOffset + (kNoAllocMaxBuckets / 2048) * 8 <= kNoAllocRedoOffset, "tiles overlap the queue");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};  // the device address of `scratch` on each GPU
+
+// Look up this GPU's copy of the array once, with the device current (pp_context_create).  The array's name in host
+// code is only a stand-in for the symbol; kernels and copies take the address found here.
+pp_status bind_noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return PP_OK;  // that GPU's calls of the entry point are refused
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return PP_OK;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return PP_ERR_DRIVER;
+    }
+    g_scratch_ptr[device] = (unsigned char*)ptr;
+    return PP_OK;
+}
+
+unsigned char* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
+
+// Whether settle_device_bits would leave the sum as it is (no refinement or coarsening, which copy and allocate).
+bool device_bits_settled(const pp_sum* s) {
+    const uint64_t target = s->ctx->tuning.bucket_target;
+    int want = 0;
+    while (want < s->max_bits && s->len > target * (1ull << want)) ++want;
+    const bool coarsen = s->len * 8 < target * (1ull << s->device_bits) && want < s->device_bits;
+    return !(want > s->device_bits || coarsen || 2ull * s->max_bucket > kMaxPairRows);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------------------------------------------
 
@@ -1352,6 +1424,11 @@ pp_status pp_context_create(int device, pp_context** out) {
     ctx->tuning.groups_per_cu = default_groups_per_cu();
     ctx->tuning.join_block = default_join_block();
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return PP_ERR_DRIVER;
+    }
+    if (bind_noalloc_scratch(device) != PP_OK) {
+        (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return PP_ERR_DRIVER;
     }
@@ -1768,6 +1845,55 @@ pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* ch, i
         r.src_max_bucket = src->max_bucket;
         ctx->capture_log.push_back(r);
     }
+    return PP_OK;
+}
+
+pp_status pp_apply_rotation_to_noalloc(pp_sum* src, pp_sum* dst, const pp_channel* ch, int adjoint,
+                                       const pp_policy* policy) {
+    if (!src || !dst || !ch || src == dst) return PP_ERR_INVALID_ARGUMENT;
+    pp_context* ctx = src->ctx;
+    PP_NOT_CAPTURING(ctx, "pp_apply_rotation_to_noalloc");
+    if (dst->ctx != ctx || dst->W != src->W || dst->kind != src->kind || dst->num_qubits != src->num_qubits ||
+        dst->seed != src->seed)
+        return fail(ctx, PP_ERR_INVALID_ARGUMENT, "pp_apply_rotation_to_noalloc: the sums differ in shape or context");
+    if (!valid_policy(policy)) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "malformed keep program");
+    unsigned char* base = noalloc_scratch(ctx->device);
+    if (!base) return fail(ctx, PP_ERR_UNSUPPORTED, "pp_apply_rotation_to_noalloc: no scratch on this device");
+    pp_status st;
+    if ((st = refresh(src)) != PP_OK) return st;  // a read-back or an in-place retain: neither allocates
+    if (!device_bits_settled(src))
+        return fail(ctx, PP_ERR_UNSUPPORTED,
+                    "pp_apply_rotation_to_noalloc: the input's device bucketing must be refined first, which "
+                    "allocates (one pp_apply_rotation_to from it does so)");
+    const uint32_t B = 1u << src->device_bits;
+    if (B > kNoAllocMaxBuckets)
+        return fail(ctx, PP_ERR_INVALID_ARGUMENT,
+                    "pp_apply_rotation_to_noalloc: " + std::to_string(B) + " device buckets exceed the scratch's " +
+                        std::to_string(kNoAllocMaxBuckets));
+    const Prepared p = prepare(src, *ch, adjoint != 0);
+    if (p.key_preserving)
+        return fail(ctx, PP_ERR_UNSUPPORTED, "pp_apply_rotation_to_noalloc: a key-preserving channel runs in place");
+    const int out = dst->cur;
+    if (!dst->col[out].x || dst->col[out].cap < join_output_rows(src) || !dst->bk[out].start || dst->bk[out].cap < B)
+        return fail(ctx, PP_ERR_INVALID_ARGUMENT,
+                    "pp_apply_rotation_to_noalloc: the output has room for " + std::to_string(dst->col[out].cap) +
+                        " rows and " + std::to_string(dst->bk[out].cap) + " buckets; this layer needs " +
+                        std::to_string(join_output_rows(src)) + " and " + std::to_string(B) +
+                        " (size it with one pp_apply_rotation_to of the same input)");
+    dst->pending_topn = false;
+    dst->pending_flags = nullptr;
+    dst->pending_score = nullptr;
+    dst->phys_len = 0;
+    dst->stale = false;
+    dst->device_bits = src->device_bits;
+    dst->reference_bits = src->reference_bits;
+    const JoinScratch fixed{(uint64_t*)(base + kNoAllocTilesOffset),
+                            (unsigned long long*)(base + kNoAllocStatsOffset), (uint32_t*)(base + kNoAllocRedoOffset)};
+    if ((st = launch_join(src, dst, out, p, lower_policy(policy), &fixed)) != PP_OK) return st;
+    // the counters where pp_sum_len(dst) reads them, in stream order
+    PP_HIP(ctx, hipMemcpyAsync(dst->d_stats, fixed.stats, kStatSlots * sizeof(unsigned long long),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    dst->stale = true;
     return PP_OK;
 }
 

@@ -150,6 +150,7 @@ def lib() -> ctypes.CDLL:
     L.pp_sum_refine.argtypes = [vp]
     L.pp_sum_coarsen_to.argtypes = [vp, i32]
     L.pp_stream_probe.argtypes = [vp, u64, u32, ctypes.POINTER(ctypes.c_double)]
+    L.pp_apply_rotation_to_noalloc.argtypes = [vp, vp, ctypes.POINTER(Channel), ctypes.c_int, ctypes.POINTER(Policy)]
     L.pp_hash_rows.argtypes = [u32, u64, u32, vp, vp]
     L.pp_hash_rows.restype = i32
     for name in ("pp_context_create", "pp_sync", "pp_context_stream", "pp_sum_upload", "pp_sum_clone",
@@ -160,7 +161,7 @@ def lib() -> ctypes.CDLL:
                  "pp_batch_propagate",
                  "pp_expectation_product", "pp_profile_enable", "pp_profile_read", "pp_profile_read_score",
                  "pp_prepare_channel", "pp_apply_prepared", "pp_apply_run", "pp_finalize_layer", "pp_settle",
-                 "pp_sum_refine", "pp_sum_coarsen_to", "pp_stream_probe"):
+                 "pp_sum_refine", "pp_sum_coarsen_to", "pp_stream_probe", "pp_apply_rotation_to_noalloc"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

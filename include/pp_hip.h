@@ -107,6 +107,10 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
                         pp_sum** out);
 /* A device-side copy (same context), e.g. to reset a benchmark input without a host round trip. */
 pp_status pp_sum_clone(const pp_sum* src, pp_sum** out);
+/* src copied into dst's own memory (same context, width, coefficient type, qubit count and hash seed): dst's terms are
+ * discarded and it takes src's layout; its columns grow only when too small.  To reset a working sum from a resident
+ * input many times (an in-place layer per reset) with no allocation after the first.  Asynchronous. */
+pp_status pp_sum_copy(pp_sum* dst, pp_sum* src);
 pp_status pp_sum_free(pp_sum* sum);
 pp_status pp_sum_len(pp_sum* sum, uint64_t* len);
 /* Download in canonical order: the reference's bucket index ascending, then lexicographic (x, z)
@@ -312,7 +316,7 @@ pp_status pp_propagate_learned(pp_sum* sum, const pp_channel* channels, uint64_t
 /* The device path's sizes, per context.  No counterpart in the reference, which fixes its device sizes
  * (records_per_block = 4096, target_bucket_len = 1024: engine/gpu/layer/options.rs:11, pauli_sum/storage.rs:54);
  * results do not depend on them.  Defaults come from PP_DEVICE_BUCKET_LEN, PP_JOIN_STAGE_ROWS and
- * PP_JOIN_GROUPS_PER_CU, PP_JOIN_BLOCK when set. */
+ * PP_JOIN_GROUPS_PER_CU, PP_JOIN_BLOCK and PP_JOIN_IN_PLACE when set. */
 typedef struct {
     uint32_t bucket_target; /* rows per device bucket, 16..512 (default 256) */
     uint32_t stage_rows;    /* anticommuting rows a join workgroup stages in its first launch, 64..1024 (a pair
@@ -321,6 +325,9 @@ typedef struct {
     uint32_t groups_per_cu; /* the join's grid: 0xFFFFFFFF one workgroup per pair (default); 0 a persistent
                                grid of as many as fit; 1..32 persistent with at most that many per CU */
     uint32_t join_block;    /* threads per join workgroup, the rows of one pass: 256 (default) or 512 */
+    uint32_t join_in_place; /* 1 (default): a layer of pp_propagate / pp_apply_rotation / pp_apply_prepared runs in
+                               place when the sum's segments have room for it (no pending TopN selection, not a
+                               Clifford arm), writing only the rows it changes; 0: always into the spare columns */
 } pp_tuning;
 pp_status pp_get_tuning(pp_context* ctx, pp_tuning* out);
 pp_status pp_set_tuning(pp_context* ctx, const pp_tuning* tuning);

@@ -134,7 +134,9 @@ static void T(clone_buckets)(const ppo_sum* src, ppo_sum* dst) {
     const T(bucket)* a = (const T(bucket)*)src->buckets;
     T(bucket)* bk = (T(bucket)*)calloc(nb, sizeof(T(bucket)));
     for (size_t b = 0; b < nb; ++b) {
-        T(bucket_reserve)(&bk[b], a[b].len ? a[b].len : 1);
+        /* half again as much room: a layer's merge swaps the bucket's storage into a worker's scratch, which then
+         * holds the next pair's output without growing (a realloc per pair serializes the workers on the allocator) */
+        T(bucket_reserve)(&bk[b], a[b].len + a[b].len / 2 + 16);
         memcpy(bk[b].k, a[b].k, (size_t)a[b].len * sizeof(T(key)));
         memcpy(bk[b].c, a[b].c, (size_t)a[b].len * sizeof(REAL));
         bk[b].len = a[b].len;
@@ -172,7 +174,8 @@ static void T(from_sorted)(ppo_sum* s, size_t n, const T(key) * keys, const REAL
         dest[i] = T(bucket_of)(s, &keys[i], s->bits);
         counts[dest[i]]++;
     }
-    for (size_t b = 0; b < nb; ++b) T(bucket_reserve)(&bk[b], counts[b]);
+    /* half again as much room as the rows, as in clone_buckets */
+    for (size_t b = 0; b < nb; ++b) T(bucket_reserve)(&bk[b], counts[b] + counts[b] / 2 + 16);
     for (size_t i = 0; i < n; ++i) {
         T(bucket)* b = &bk[dest[i]];
         b->k[b->len] = keys[i];
@@ -240,7 +243,7 @@ static void T(refine)(ppo_sum* s) {
                 f ^= (src->k[i].x[w] & s->rows_x[new_bit][w]) ^ (src->k[i].z[w] & s->rows_z[new_bit][w]);
             nhi += (uint32_t)(__builtin_popcountll(f) & 1);
         }
-        T(bucket_reserve)(hi, nhi);
+        T(bucket_reserve)(hi, nhi + nhi / 2 + 16);
         /* the lower half stays in the source buffer, compacted in order */
         *lo = *src;
         uint32_t keep = 0;

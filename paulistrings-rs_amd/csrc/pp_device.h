@@ -18,6 +18,7 @@ constexpr int kMaxKeepNodes = 15;
 // different XCDs with their own L2, would each move that line.
 constexpr uint32_t kSegAlign = 16;
 constexpr int kMaxHashRows = 31;
+constexpr uint32_t kPairRowsMax = 1024;  // the largest bucket pair a join stages (buckets are refined below it)
 
 enum KeepOp : int32_t { KEEP_TRUE = 0, KEEP_COEFF = 1, KEEP_WEIGHT = 2, KEEP_AND = 3, KEEP_OR = 4 };
 

@@ -64,6 +64,12 @@ uint32_t default_join_block() {
     return e && std::strtol(e, nullptr, 10) == 512 ? 512u : 256u;
 }
 
+// Layers in place when the segments have room (PP_JOIN_IN_PLACE=0 turns it off; default on).
+uint32_t default_join_in_place() {
+    const char* e = std::getenv("PP_JOIN_IN_PLACE");
+    return e && std::strcmp(e, "0") == 0 ? 0u : 1u;
+}
+
 // Rows per device bucket (PP_DEVICE_BUCKET_LEN, 16..512; default 256).
 uint32_t default_bucket_target() {
     const char* e = std::getenv("PP_DEVICE_BUCKET_LEN");
@@ -74,7 +80,7 @@ uint32_t default_bucket_target() {
 bool valid_tuning(const pp_tuning& t) {
     return t.bucket_target >= 16 && t.bucket_target <= 512 && (t.stage_rows == 0 || (t.stage_rows >= 64 && t.stage_rows <= 1024)) &&
            (t.groups_per_cu <= 32 || t.groups_per_cu == kJoinOnePairPerGroup) &&
-           (t.join_block == 256 || t.join_block == 512);
+           (t.join_block == 256 || t.join_block == 512) && t.join_in_place <= 1;
 }
 
 // The staging rows of a join whose largest pair is lcap rows: the tuning's (or lcap), a multiple of 64, at most lcap.
@@ -267,6 +273,7 @@ struct pp_context {
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     uint64_t prof_rows_in = 0;
+    uint64_t prof_rows_out = 0;  // output rows of the profiled layers the host counted (in-place layers)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> score_events;  // the learned pass's score kernel
     uint64_t prof_scored = 0;
     unsigned long long* d_prof = nullptr;  // [0]: output rows of the profiled joins
@@ -289,6 +296,7 @@ struct Columns {
 struct BucketArrays {
     uint64_t* start = nullptr;
     uint32_t* len = nullptr;
+    uint32_t* rows = nullptr;  // each segment's room in rows (valid when the sum's has_room says so)
     uint64_t cap = 0;
 };
 
@@ -325,6 +333,10 @@ struct pp_sum {
     uint64_t len = 0;
     uint32_t max_bucket = 0;
     bool stale = false;  // len and max_bucket await the last layer's counters (pp_apply_rotation_to)
+    // The current column set's free room: its segments' rooms (bk[cur].rows) are valid, and every row at or after
+    // `tail` is free.  Layouts from an offset scan, an upload, a refinement of one, and in-place layers keep both.
+    bool has_room = false;
+    uint64_t tail = 0;
     // A TopN selection whose retain is left to the next join (inside pp_propagate only): len is then the kept
     // count, phys_len the rows the columns still hold, and max_bucket their longest bucket.
     bool pending_topn = false;
@@ -398,6 +410,7 @@ void free_columns(Columns& c) {
 void free_buckets(BucketArrays& b) {
     if (b.start) (void)hipFree(b.start);
     if (b.len) (void)hipFree(b.len);
+    if (b.rows) (void)hipFree(b.rows);
     b = BucketArrays{};
 }
 
@@ -423,6 +436,7 @@ pp_status ensure_buckets(pp_sum* s, int which, uint64_t need) {
     pp_status st;
     if ((st = dev_alloc(s->ctx, (void**)&b.start, cap * 8)) != PP_OK) return st;
     if ((st = dev_alloc(s->ctx, (void**)&b.len, cap * 4)) != PP_OK) return st;
+    if ((st = dev_alloc(s->ctx, (void**)&b.rows, cap * 4)) != PP_OK) return st;
     b.cap = cap;
     return PP_OK;
 }
@@ -497,20 +511,23 @@ pp_status slot_scratch(pp_sum* s, ScratchSlot slot, size_t bytes, P** out) {
 
 // With clear_counters the scan also zeroes the sum's counters (before it writes the total into slot 3), in place of
 // clear_stats.
+// out_rows (a resident layout's segment rooms) is written with the starts; the total lands in counter slot 3.
 pp_status offsets(pp_sum* s, const uint32_t* len, uint32_t num_buckets, uint32_t mode, uint32_t delta,
-                  uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start, bool clear_counters = false) {
+                  uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start, bool clear_counters = false,
+                  uint32_t* out_rows = nullptr) {
     pp_status st;
     if ((st = ensure_tiles(s, mode == 2 ? coarse_buckets : num_buckets)) != PP_OK) return st;
     PP_HIP(s->ctx, launch_bucket_offsets(s->ctx->stream, len, num_buckets, mode, delta, coarse_buckets, fold,
                                          out_start, s->d_stats + 3, s->d_tiles,
-                                         clear_counters ? s->d_stats : nullptr, kStatSlots));
+                                         clear_counters ? s->d_stats : nullptr, kStatSlots, out_rows));
     return PP_OK;
 }
 
-// len and max_bucket of the current layout, read back (one synchronization).
+// len and max_bucket of the current layout, read back (one synchronization).  Counter slot 3 (the last offset
+// scan's total, the end of the layout it made) is kept.
 pp_status count_layout(pp_sum* s, int which, uint32_t num_buckets) {
     pp_status st;
-    if ((st = clear_stats(s)) != PP_OK) return st;
+    PP_HIP(s->ctx, launch_fill_u64(s->ctx->stream, s->d_stats, 3, 0ull));
     PP_HIP(s->ctx, launch_reduce_lens(s->ctx->stream, s->bk[which].len, num_buckets, s->d_stats, nullptr));
     if ((st = read_stats(s)) != PP_OK) return st;
     s->len = s->h_stats[0];
@@ -547,6 +564,8 @@ pp_status refine_once(pp_sum* s) {
         a.row = s->device_bits;
         a.rows = s->d_rows;
         a.stats = s->d_stats;
+        a.bcap = s->has_room ? s->bk[in].rows : nullptr;  // the halves share the segment's room (the tail stays)
+        a.obcap = s->has_room ? s->bk[out].rows : nullptr;
         return launch_refine<W, T>(s->ctx->stream, a);
     });
     PP_HIP(s->ctx, e);
@@ -562,7 +581,8 @@ pp_status coarsen_to(pp_sum* s, int bits) {
     pp_status st;
     if ((st = ensure_columns(s, out, s->len + (uint64_t)kSegAlign * CB)) != PP_OK) return st;
     if ((st = ensure_buckets(s, out, CB)) != PP_OK) return st;
-    if ((st = offsets(s, s->bk[in].len, B, 2, 0, CB, B / CB, s->bk[out].start)) != PP_OK) return st;
+    if ((st = offsets(s, s->bk[in].len, B, 2, 0, CB, B / CB, s->bk[out].start, false, s->bk[out].rows)) != PP_OK)
+        return st;
     hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
         constexpr int W = decltype(Wc)::value;
         using T = decltype(Tv);
@@ -586,6 +606,8 @@ pp_status coarsen_to(pp_sum* s, int bits) {
     });
     PP_HIP(s->ctx, e);
     if ((st = count_layout(s, out, CB)) != PP_OK) return st;
+    s->tail = s->h_stats[3];
+    s->has_room = true;
     s->device_bits = bits;
     s->cur = out;
     return PP_OK;
@@ -991,8 +1013,10 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, cons
     // the scan also clears dst's counters
     if (fixed) {
         PP_HIP(ctx, launch_bucket_offsets(ctx->stream, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start,
-                                          fixed->stats + 3, fixed->tiles, fixed->stats, kStatSlots));
-    } else if ((st = offsets(dst, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start, true)) != PP_OK) {
+                                          fixed->stats + 3, fixed->tiles, fixed->stats, kStatSlots,
+                                          dst->bk[out].rows));
+    } else if ((st = offsets(dst, src->bk[in].len, B, 1, delta, 0, 0, dst->bk[out].start, true,
+                             dst->bk[out].rows)) != PP_OK) {
         return st;
     }
     // lcap bounds the largest pair; the first launch stages scap anticommuting rows per pair, and the pairs with
@@ -1087,6 +1111,154 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, cons
     return PP_OK;
 }
 
+// Whether a join layer runs in place (k_rotation_inplace) rather than into the spare columns: the layout's segment
+// rooms are known, no TopN selection is pending (every row it drops would be a hole to fill), the layer is not a
+// Clifford arm (every anticommuting row would be one) and earlier moves have not left more dead rows than live ones.
+bool join_in_place(const pp_sum* s, const Prepared& p) {
+    if (!s->ctx->tuning.join_in_place || !s->has_room || s->pending_topn || p.clifford || s->len == 0) return false;
+    const uint64_t B = 1ull << s->device_bits;
+    return s->tail <= 3 * s->len + 4ull * kSegAlign * B;
+}
+
+// Grow column set `which` to at least `need` rows, keeping rows [0, keep) where they are (the bucket arrays stay
+// valid).
+pp_status grow_columns_keeping(pp_sum* s, int which, uint64_t need, uint64_t keep) {
+    Columns& c = s->col[which];
+    if (c.cap >= need) return PP_OK;
+    Columns old = c;
+    c = Columns{};
+    pp_status st = ensure_columns(s, which, need);
+    if (st != PP_OK) {
+        free_columns(c);
+        c = old;
+        return st;
+    }
+    hipStream_t stream = s->ctx->stream;
+    for (uint32_t w = 0; w < s->W; ++w) {
+        PP_HIP(s->ctx, hipMemcpyAsync(c.x + w * c.cap, old.x + w * old.cap, keep * 8, hipMemcpyDeviceToDevice, stream));
+        PP_HIP(s->ctx, hipMemcpyAsync(c.z + w * c.cap, old.z + w * old.cap, keep * 8, hipMemcpyDeviceToDevice, stream));
+    }
+    PP_HIP(s->ctx, hipMemcpyAsync(c.c, old.c, keep * s->coeff_bytes(), hipMemcpyDeviceToDevice, stream));
+    PP_HIP(s->ctx, hipStreamSynchronize(stream));
+    free_columns(old);
+    return PP_OK;
+}
+
+// A join layer in place: the first launch does every pair that fits its segments and queues the others untouched;
+// then the columns are grown if the queued pairs' moves need it, and the redo launch does the queued pairs, moving
+// those that do not fit to fresh segments after the last used row.
+pp_status layer_in_place(pp_sum* s, const Prepared& p, const KeepProgram& keep) {
+    pp_context* ctx = s->ctx;
+    pp_status st;
+    const int cur = s->cur;
+    const uint32_t B = 1u << s->device_bits;
+    const uint32_t delta = bucket_of_host(s, p.gx, p.gz, s->device_bits);
+    const uint32_t cosets = delta ? B / 2 : B;
+    uint32_t lcap = delta ? 2u * s->max_bucket : s->max_bucket;
+    lcap = std::max<uint32_t>(64, (lcap + 63) & ~63u);
+    if (lcap > kMaxPairRows) lcap = kMaxPairRows;
+    // a generator anticommutes with a fraction of a bucket: 256 staged rows leave room for six workgroups per CU
+    const uint32_t scap = std::min<uint32_t>(ctx->tuning.stage_rows ? join_stage_rows(ctx->tuning.stage_rows, lcap)
+                                                                     : 256u,
+                                             lcap);
+    auto table_bits_for = [](uint32_t rows) {
+        uint32_t b = 1;
+        while ((1u << b) < 2u * rows) ++b;
+        return b;
+    };
+    if (s->redo_cap < B) {
+        if (s->d_redo) (void)hipFree(s->d_redo);
+        s->d_redo = nullptr;
+        s->redo_cap = 0;
+        if ((st = dev_alloc(ctx, (void**)&s->d_redo, (size_t)B * 4)) != PP_OK) return st;
+        s->redo_cap = B;
+    }
+    PP_HIP(ctx, launch_fill_u64(ctx->stream, s->d_stats, kStatSlots, 0ull));
+    const bool timed = ctx->profiling && !ctx->capturing;
+    constexpr uint32_t kCountPairs = 512;
+    const bool count_in_join = !timed && cosets <= kCountPairs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (timed) {
+        PP_HIP(ctx, hipEventCreate(&ev0));
+        PP_HIP(ctx, hipEventCreate(&ev1));
+        PP_HIP(ctx, hipEventRecord(ev0, ctx->stream));
+    }
+    auto launch = [&](uint32_t redo_mode, uint32_t stage) {
+        return dispatch(s, [&](auto Wc, auto Tv) {
+            constexpr int W = decltype(Wc)::value;
+            using T = decltype(Tv);
+            InplaceArgs<W, T> a{};
+            a.x = s->col[cur].x;
+            a.z = s->col[cur].z;
+            a.c = (T*)s->col[cur].c;
+            a.cap = s->col[cur].cap;
+            a.start = s->bk[cur].start;
+            a.len = s->bk[cur].len;
+            a.bcap = s->bk[cur].rows;
+            a.tail_base = s->tail;
+            a.tail_limit = s->col[cur].cap;
+            a.delta = delta;
+            a.hb = delta ? 31u - (uint32_t)__builtin_clz(delta) : 0u;
+            a.scap = stage;
+            a.tcap_bits = table_bits_for(stage);
+            a.num_cosets = cosets;
+            uint32_t gphase = 0;
+            for (uint32_t w = 0; w < W; ++w) {
+                a.gx[w] = p.gx[w];
+                a.gz[w] = p.gz[w];
+                gphase += (uint32_t)__builtin_popcountll(p.gx[w] & p.gz[w]);
+            }
+            a.gphase = gphase & 3u;
+            a.cos_t = (T)p.cos;
+            a.sin_t = (T)p.sin;
+            a.clifford = p.clifford ? 1 : 0;
+            a.keep = keep;
+            a.stats = s->d_stats;
+            a.redo = s->d_redo;
+            a.redo_count = s->d_stats + 4;
+            a.redo_mode = redo_mode;
+            a.count_totals = count_in_join ? 1u : 0u;
+            return launch_rotation_inplace<W, T>(ctx->stream, a);
+        });
+    };
+    // the lengths come from the workgroups' own totals (few pairs) or one reduce over the buckets
+    auto count = [&]() -> pp_status {
+        if (!count_in_join) PP_HIP(ctx, launch_reduce_lens(ctx->stream, s->bk[cur].len, B, s->d_stats, nullptr));
+        return read_stats(s);
+    };
+    PP_HIP(ctx, launch(0, scap));
+    if (timed) {
+        PP_HIP(ctx, hipEventRecord(ev1, ctx->stream));
+        ctx->events.push_back({ev0, ev1});
+        ctx->prof_rows_in += s->len;
+    }
+    if ((st = count()) != PP_OK) return st;  // the one synchronization when every pair fits
+    if (s->h_stats[4]) {  // queued pairs: room for their moves, then the redo launch, then the lengths again
+        if ((st = grow_columns_keeping(s, cur, s->tail + s->h_stats[7], s->tail)) != PP_OK) return st;
+        if (!count_in_join) PP_HIP(ctx, launch_fill_u64(ctx->stream, s->d_stats, 2, 0ull));
+        hipEvent_t r0 = nullptr, r1 = nullptr;
+        if (timed) {
+            PP_HIP(ctx, hipEventCreate(&r0));
+            PP_HIP(ctx, hipEventCreate(&r1));
+            PP_HIP(ctx, hipEventRecord(r0, ctx->stream));
+        }
+        PP_HIP(ctx, launch(1, lcap));
+        if (timed) {
+            PP_HIP(ctx, hipEventRecord(r1, ctx->stream));
+            ctx->events.push_back({r0, r1});
+        }
+        if ((st = count()) != PP_OK) return st;
+    }
+    if (s->h_stats[2])
+        return fail(ctx, PP_ERR_DRIVER,
+                    s->h_stats[2] & 2 ? "in-place layer: no room to move a pair" : "in-place layer: staging overflow");
+    s->len = s->h_stats[0];
+    s->max_bucket = (uint32_t)s->h_stats[1];
+    s->tail += s->h_stats[5];
+    if (timed) ctx->prof_rows_out += s->len;
+    return PP_OK;
+}
+
 pp_status device_layer(pp_sum* s, const Prepared& p, const KeepProgram& keep) {
     pp_context* ctx = s->ctx;
     pp_status st;
@@ -1116,6 +1288,7 @@ pp_status device_layer(pp_sum* s, const Prepared& p, const KeepProgram& keep) {
         PP_HIP(ctx, e);
         return count_layout(s, s->cur, B);
     }
+    if (join_in_place(s, p)) return layer_in_place(s, p, keep);
     const int out = 1 - s->cur;
     if ((st = launch_join(s, s, out, p, keep)) != PP_OK) return st;
     s->pending_topn = false;  // the join read its input through the selection
@@ -1123,6 +1296,8 @@ pp_status device_layer(pp_sum* s, const Prepared& p, const KeepProgram& keep) {
     if (s->h_stats[2]) return fail(ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
     s->len = s->h_stats[0];
     s->max_bucket = (uint32_t)s->h_stats[1];
+    s->tail = s->h_stats[3];
+    s->has_room = true;
     s->cur = out;
     return PP_OK;
 }
@@ -1184,7 +1359,7 @@ pp_status permutation_run_device(pp_sum* s, const Prepared* steps, uint32_t n_st
         PP_HIP(ctx, e);
         if ((st = ensure_columns(s, out, s->len + (uint64_t)kSegAlign * B)) != PP_OK) return st;
         if ((st = ensure_buckets(s, out, B)) != PP_OK) return st;
-        if ((st = offsets(s, d_counts, B, 3, 0, 0, 0, s->bk[out].start)) != PP_OK) return st;
+        if ((st = offsets(s, d_counts, B, 3, 0, 0, 0, s->bk[out].start, false, s->bk[out].rows)) != PP_OK) return st;
         PP_HIP(ctx, hipMemsetAsync(s->bk[out].len, 0, (size_t)B * 4, ctx->stream));
         e = dispatch(s, [&](auto Wc, auto Tv) {
             constexpr int W = decltype(Wc)::value;
@@ -1209,6 +1384,8 @@ pp_status permutation_run_device(pp_sum* s, const Prepared* steps, uint32_t n_st
         PP_HIP(ctx, hipMemcpyAsync(deaths.data() + first, d_deaths, m * 8, hipMemcpyDeviceToHost, ctx->stream));
         s->cur = out;
         if ((st = count_layout(s, s->cur, B)) != PP_OK) return st;  // synchronizes
+        s->tail = s->h_stats[3];
+        s->has_room = true;
     }
     uint64_t len = len_in;
     for (uint32_t j = 0; j < n_steps; ++j) {
@@ -1230,6 +1407,8 @@ pp_status refresh_counts(pp_sum* s) {
     if (s->h_stats[2]) return fail(s->ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
     s->len = s->h_stats[0];
     s->max_bucket = (uint32_t)s->h_stats[1];
+    s->tail = s->h_stats[3];  // the out-of-place layer's offset scan laid the segments out up to here
+    s->has_room = true;
     return PP_OK;
 }
 
@@ -1238,13 +1417,7 @@ pp_status refresh_counts(pp_sum* s) {
 pp_status refresh(pp_sum* s) {
     pp_status st;
     if (s->pending_topn && (st = materialize_topn(s)) != PP_OK) return st;
-    if (!s->stale) return PP_OK;
-    if ((st = read_stats(s)) != PP_OK) return st;
-    s->stale = false;
-    if (s->h_stats[2]) return fail(s->ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
-    s->len = s->h_stats[0];
-    s->max_bucket = (uint32_t)s->h_stats[1];
-    return PP_OK;
+    return refresh_counts(s);
 }
 
 pp_status apply_one(pp_sum* s, const Prepared& p, const KeepProgram& keep) { return device_layer(s, p, keep); }
@@ -1423,6 +1596,7 @@ pp_status pp_context_create(int device, pp_context** out) {
     ctx->tuning.stage_rows = default_stage_rows();
     ctx->tuning.groups_per_cu = default_groups_per_cu();
     ctx->tuning.join_block = default_join_block();
+    ctx->tuning.join_in_place = default_join_in_place();
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return PP_ERR_DRIVER;
@@ -1542,14 +1716,16 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
     std::vector<uint64_t> order, bstart;
     parallel_counting_sort(bucket.data(), m, B, T, order, bstart);
     std::vector<uint64_t> start(B);
-    std::vector<uint32_t> lens(B);
+    std::vector<uint32_t> lens(B), room(B);
     uint64_t acc = 0;
     uint32_t maxb = 0;
     for (uint32_t b = 0; b < B; ++b) {
         const uint32_t cnt = (uint32_t)(bstart[b + 1] - bstart[b]);
         start[b] = acc;
         lens[b] = cnt;
-        acc += (cnt + (kSegAlign - 1)) & ~(kSegAlign - 1);
+        // a quarter more room, for the partners a first layer in place adds to the bucket
+        room[b] = (cnt + cnt / 4 + (kSegAlign - 1)) & ~(kSegAlign - 1);
+        acc += room[b];
         maxb = std::max(maxb, cnt);
     }
     s->max_bucket = maxb;
@@ -1580,9 +1756,12 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
         hipMemcpy(s->col[0].z, hz.get(), (size_t)width * cap * 8, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->col[0].c, hc.get(), cap * cb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->bk[0].start, start.data(), B * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->bk[0].len, lens.data(), B * 4, hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(s->bk[0].len, lens.data(), B * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->bk[0].rows, room.data(), B * 4, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(ctx, PP_ERR_DRIVER, "upload copy failed"));
     s->cur = 0;
+    s->tail = acc;
+    s->has_room = true;
     *out = s;
     return PP_OK;
 }
@@ -1635,6 +1814,8 @@ pp_status pp_sum_clone(const pp_sum* src_c, pp_sum** out) {
     s->reference_bits = src->reference_bits;
     s->len = src->len;
     s->max_bucket = src->max_bucket;
+    s->has_room = src->has_room;
+    s->tail = src->tail;
     auto bail = [&](pp_status st) {
         pp_sum_free(s);
         return st;
@@ -1662,11 +1843,58 @@ pp_status pp_sum_clone(const pp_sum* src_c, pp_sum** out) {
     if (hipMemcpyAsync(s->col[0].c, sc.c, sc.cap * s->coeff_bytes(), hipMemcpyDeviceToDevice, stream) != hipSuccess ||
         hipMemcpyAsync(s->bk[0].start, sb.start, B * 8, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
         hipMemcpyAsync(s->bk[0].len, sb.len, B * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+        (src->has_room &&
+         hipMemcpyAsync(s->bk[0].rows, sb.rows, B * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess) ||
         hipMemcpyAsync(s->d_rows, s->rows.data(), s->rows.size() * 8, hipMemcpyHostToDevice, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
         return bail(fail(ctx, PP_ERR_DRIVER, "clone copy failed"));
     s->cur = 0;
     *out = s;
+    return PP_OK;
+}
+
+pp_status pp_sum_copy(pp_sum* dst, pp_sum* src) {
+    if (src) PP_NOT_CAPTURING(src->ctx, "pp_sum_copy");
+    if (!dst || !src || dst == src) return PP_ERR_INVALID_ARGUMENT;
+    pp_context* ctx = src->ctx;
+    if (dst->ctx != ctx || dst->W != src->W || dst->kind != src->kind || dst->num_qubits != src->num_qubits ||
+        dst->seed != src->seed)
+        return fail(ctx, PP_ERR_INVALID_ARGUMENT, "pp_sum_copy: the sums differ in shape or context");
+    pp_status st;
+    if ((st = refresh(src)) != PP_OK) return st;
+    for (const pp_graph* g : ctx->graphs)
+        for (const CapturedJoin& r : g->layers)
+            if (r.dst == dst) return fail(ctx, PP_ERR_INVALID_ARGUMENT, "pp_sum_copy: a live graph writes dst");
+    const int d = dst->cur;
+    const Columns& sc = src->col[src->cur];
+    const BucketArrays& sb = src->bk[src->cur];
+    const uint64_t B = 1ull << src->device_bits;
+    // every row a segment of src may hold: the tail when known, else the whole column
+    const uint64_t rows = src->has_room ? src->tail : sc.cap;
+    if ((st = ensure_columns(dst, d, rows)) != PP_OK) return st;
+    if ((st = ensure_buckets(dst, d, B)) != PP_OK) return st;
+    const Columns& dc = dst->col[d];
+    const BucketArrays& db = dst->bk[d];
+    hipStream_t stream = ctx->stream;
+    for (uint32_t w = 0; w < src->W; ++w) {
+        PP_HIP(ctx, hipMemcpyAsync(dc.x + w * dc.cap, sc.x + w * sc.cap, rows * 8, hipMemcpyDeviceToDevice, stream));
+        PP_HIP(ctx, hipMemcpyAsync(dc.z + w * dc.cap, sc.z + w * sc.cap, rows * 8, hipMemcpyDeviceToDevice, stream));
+    }
+    PP_HIP(ctx, hipMemcpyAsync(dc.c, sc.c, rows * src->coeff_bytes(), hipMemcpyDeviceToDevice, stream));
+    PP_HIP(ctx, hipMemcpyAsync(db.start, sb.start, B * 8, hipMemcpyDeviceToDevice, stream));
+    PP_HIP(ctx, hipMemcpyAsync(db.len, sb.len, B * 4, hipMemcpyDeviceToDevice, stream));
+    if (src->has_room) PP_HIP(ctx, hipMemcpyAsync(db.rows, sb.rows, B * 4, hipMemcpyDeviceToDevice, stream));
+    dst->pending_topn = false;
+    dst->pending_flags = nullptr;
+    dst->pending_score = nullptr;
+    dst->phys_len = 0;
+    dst->stale = false;
+    dst->len = src->len;
+    dst->max_bucket = src->max_bucket;
+    dst->device_bits = src->device_bits;
+    dst->reference_bits = src->reference_bits;
+    dst->has_room = src->has_room;
+    dst->tail = src->tail;
     return PP_OK;
 }
 
@@ -2105,6 +2333,7 @@ pp_status pp_profile_enable(pp_context* ctx, int enable) {
     }
     ctx->events.clear();
     ctx->prof_rows_in = 0;
+    ctx->prof_rows_out = 0;
     for (auto& e : ctx->score_events) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
@@ -2137,7 +2366,7 @@ pp_status pp_profile_read(pp_context* ctx, uint64_t* launches, double* total_ms,
     if (rows_in) *rows_in = ctx->prof_rows_in;
     unsigned long long out_rows = 0;
     if (ctx->d_prof) PP_HIP(ctx, hipMemcpy(&out_rows, ctx->d_prof, sizeof(out_rows), hipMemcpyDeviceToHost));
-    if (rows_out) *rows_out = out_rows;
+    if (rows_out) *rows_out = out_rows + ctx->prof_rows_out;
     return PP_OK;
 }
 
@@ -2356,6 +2585,7 @@ pp_status batch_step(pp_batch* b, const std::vector<uint32_t>& idx, const std::v
         s->len = b->h_out[4 * k];
         s->max_bucket = (uint32_t)b->h_out[4 * k + 1];
         s->cur = 1 - s->cur;
+        s->has_room = false;  // the batched offsets lay out starts only: the next layer runs out of place
     }
     return PP_OK;
 }
@@ -2492,7 +2722,7 @@ pp_status pp_autotune(pp_sum* sample, const pp_channel* channels, uint64_t n_cha
     for (uint32_t bt : {128u, 192u, 256u, 384u})
         for (uint32_t sr : {0u, 128u, 256u})
             for (uint32_t gp : {kJoinOnePairPerGroup, 0u})
-                for (uint32_t jb : {256u, 512u}) candidates.push_back(pp_tuning{bt, sr, gp, jb});
+                for (uint32_t jb : {256u, 512u}) candidates.push_back(pp_tuning{bt, sr, gp, jb, saved.join_in_place});
     for (const pp_tuning& t : candidates) {
         double ms = 0;
         if ((st = trial(t, &ms)) != PP_OK) break;

@@ -51,6 +51,40 @@ struct JoinArgs {
     const float* filter_score;    // its ranking score per input position when it ranked by score, else null (|c|)
 };
 
+// The rotation layer in place (k_rotation_inplace): the sum's own columns and bucket arrays are read and written.
+template <int W, typename T>
+struct InplaceArgs {
+    uint64_t* x;
+    uint64_t* z;
+    T* c;
+    uint64_t cap;
+    uint64_t* start;
+    uint32_t* len;
+    uint32_t* bcap;      // each segment's rows
+    uint64_t tail_base;  // the first row after every segment: moved pairs allocate from tail_base + counter 5
+    uint64_t tail_limit; // rows of the columns: a moved pair's segments stay below it
+    uint32_t delta;
+    uint32_t hb;
+    uint32_t scap;
+    uint32_t tcap_bits;
+    uint32_t num_cosets;
+    uint32_t gphase;
+    uint64_t gx[W];
+    uint64_t gz[W];
+    T cos_t;
+    T sin_t;
+    int32_t clifford;
+    KeepProgram keep;
+    // [0] rows and [1] longest bucket when count_totals; [2] |= 1 redo overflow, |= 2 no room to move a pair;
+    // [4] queued pairs; [5] rows moved pairs took after tail_base; [6] moved pairs; [7] rows the queued pairs' moves
+    // may take
+    unsigned long long* stats;
+    uint32_t* redo;
+    unsigned long long* redo_count;
+    uint32_t redo_mode;
+    uint32_t count_totals;
+};
+
 // A local table applied row by row, one output per row (a permutation step, PermutationStep::of,
 // engine/bucketed/permutation_run.rs:53-121, or a key-preserving rescale): the key's pattern on the table's support
 // (bit 2j the x bit of support slot j, bit 2j+1 its z bit, channel/prepared.rs:141-151; a one-qubit table repeats
@@ -132,6 +166,8 @@ struct RefineArgs {
     int32_t row;           // the hash row that splits
     const uint64_t* rows;  // [row][x|z][W]
     unsigned long long* stats;
+    const uint32_t* bcap;  // the input segments' rows, or null (then no output segment rows)
+    uint32_t* obcap;
 };
 
 template <int W, typename T>
@@ -190,10 +226,11 @@ struct ExportArgs {
 // holds offsets_tiles(n) words.  When clear is set, clear[0 .. clear_n) is zeroed before the total is written (a
 // join's counters, without a launch of their own).
 uint32_t offsets_tiles(uint32_t out_n);
+// out_cap (modes 1-3) receives each resident segment's rows.
 hipError_t launch_bucket_offsets(hipStream_t st, const uint32_t* len, uint32_t num_buckets, uint32_t mode,
                                  uint32_t delta, uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start,
                                  unsigned long long* total, uint64_t* tile_sums, unsigned long long* clear = nullptr,
-                                 uint32_t clear_n = 0);
+                                 uint32_t clear_n = 0, uint32_t* out_cap = nullptr);
 // p[0 .. n) = v, as a kernel (capturable).
 hipError_t launch_fill_u64(hipStream_t st, unsigned long long* p, uint32_t n, unsigned long long v);
 // out[0] += total rows, out[1] = max(out[1], longest bucket); prof[0] += total rows when prof is given.
@@ -217,6 +254,10 @@ hipError_t launch_join_batch(hipStream_t st, const JoinArgs<W, T>* items, const 
                              unsigned long long* out);
 template <int W, typename T>
 hipError_t launch_rescale(hipStream_t st, const RescaleArgs<W, T>& a, uint32_t num_buckets);
+template <int W, typename T>
+size_t inplace_lds_bytes(uint32_t scap, uint32_t tcap_bits);
+template <int W, typename T>
+hipError_t launch_rotation_inplace(hipStream_t st, const InplaceArgs<W, T>& a);
 template <int W, typename T>
 hipError_t launch_perm_walk(hipStream_t st, const PermWalkArgs<W, T>& a, uint32_t num_buckets);
 template <int W, typename T>

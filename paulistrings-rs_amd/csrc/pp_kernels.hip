@@ -30,6 +30,7 @@ struct OffsetsArgs {
     uint32_t coarse_buckets;
     uint32_t fold;
     uint64_t* out_start;
+    uint32_t* out_cap;  // when set (a resident layout), each segment's rows
     uint64_t* tile_sums;
     unsigned long long* total;
     unsigned long long* clear;  // when set, clear[0 .. clear_n) is zeroed before *total is written
@@ -120,6 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_offsets_final(const OffsetsArgs a) {
     for (uint32_t i = 0; i < kScanItems; ++i)
         if (lo + i < a.out_n) {
             a.out_start[lo + i] = base;
+            if (a.out_cap) a.out_cap[lo + i] = (uint32_t)sz[i];
             base += sz[i];
         }
 }
@@ -142,6 +144,7 @@ __global__ void __launch_bounds__(kBlock) k_offsets_one(const OffsetsArgs a) {
     for (uint32_t i = 0; i < kScanItems; ++i)
         if (lo + i < a.out_n) {
             a.out_start[lo + i] = base;
+            if (a.out_cap) a.out_cap[lo + i] = (uint32_t)sz[i];
             base += sz[i];
         }
     if (threadIdx.x == 0) {
@@ -558,6 +561,338 @@ __global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a)
     join_items<W, T, REDO, BLOCK, FILTER>(a, blockIdx.x, gridDim.x, count, lds, scan_scratch);
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// The rotation layer in place.  A workgroup per pair {b0, b1} reads every row of the pair, as k_rotation_join does,
+// but writes only what the layer changes: the coefficient of an anticommuting row that stays, and the new partner
+// rows, which go first into the rows the layer drops on their side (holes) and then after the side's last row.  A
+// commuting row that stays is not written.  Every decision is taken before the first write.  The first launch
+// leaves a pair untouched and queues it when it stages more anticommuting rows than the workgroup holds or does not
+// fit its segments (a side with more holes than new rows, or more rows than its segment's room), adding the rows its
+// move may take to counter 7.  The redo launch (REDO) stages the largest pair: a queued pair that fits is done in
+// place, any other is written whole to fresh segments after the columns' last used row (tail_base plus counter 5,
+// which the host has made room for), its old segments left as dead space.
+// ---------------------------------------------------------------------------------------------------------------
+
+constexpr uint32_t kInplaceHoles = 1024;  // holes per side a workgroup records (a bucket holds at most 512 rows)
+
+// The rows a moved bucket's fresh segment holds: its rows, an eighth more for the next layers, rounded to kSegAlign.
+__host__ __device__ constexpr uint32_t moved_segment_rows(uint32_t rows) {
+    return (rows + rows / 8 + (kSegAlign - 1)) & ~(kSegAlign - 1);
+}
+
+template <int W, typename T>
+size_t inplace_lds_bytes(uint32_t scap, uint32_t tcap_bits) {
+    return (size_t)scap * (2 * W * sizeof(uint64_t) + 3 * sizeof(T) + 4 + 2 + 2 + 1 + 1) + ((size_t)4 << tcap_bits) +
+           2 * (size_t)kInplaceHoles * 2;
+}
+
+template <int W, typename T, bool REDO>
+__global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W, T> a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t scan_scratch[4 * kWaves];
+    __shared__ uint64_t reloc_base;
+    const uint32_t S = a.scap;
+    uint64_t* const Lx = reinterpret_cast<uint64_t*>(lds);
+    uint64_t* const Lz = Lx + (size_t)W * S;
+    T* const Lc = reinterpret_cast<T*>(Lz + (size_t)W * S);
+    T* const Lown = Lc + S;
+    T* const Lfresh = Lown + S;
+    uint32_t* const Lf = reinterpret_cast<uint32_t*>(Lfresh + S);
+    uint32_t* const table = Lf + S;
+    uint16_t* const Li = reinterpret_cast<uint16_t*>(table + (1u << a.tcap_bits));
+    uint16_t* const Lrank = Li + S;
+    uint16_t* const holes = Lrank + S;  // [2][kInplaceHoles]
+    uint8_t* const Lm = reinterpret_cast<uint8_t*>(holes + 2 * kInplaceHoles);
+    uint8_t* const Lflag = Lm + S;
+
+    const uint32_t count = REDO ? (uint32_t)*a.redo_count : a.num_cosets;
+    const bool pair = a.delta != 0;
+    Key<W> g;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        g.x[w] = a.gx[w];
+        g.z[w] = a.gz[w];
+    }
+    const uint32_t supp = support_words<W>(g);
+    const uint32_t fold_g = key_fold<W>(g);
+    for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {  // uniform
+        const uint32_t p = REDO ? a.redo[t] : t;
+        uint32_t b0, b1;
+        if (pair) {
+            const uint32_t lo = p & ((1u << a.hb) - 1u);
+            b0 = ((p >> a.hb) << (a.hb + 1u)) | lo;
+            b1 = b0 ^ a.delta;
+        } else {
+            b0 = b1 = p;
+        }
+        const uint32_t n0 = a.len[b0], n1 = pair ? a.len[b1] : 0u;
+        const uint64_t s0 = a.start[b0], s1 = a.start[b1];
+        const uint32_t cap0 = a.bcap[b0], cap1 = a.bcap[b1];
+        const uint32_t n = n0 + n1;
+        const uint32_t tbits = min(table_bits(min(max(n, 1u), S)), a.tcap_bits);
+        const uint32_t tmask = (1u << tbits) - 1u;
+        __syncthreads();  // the previous pair's LDS is no longer read
+        for (uint32_t i = threadIdx.x; i < (1u << tbits); i += kBlock) table[i] = 0u;
+        __syncthreads();
+        // ---- phase A: holes and staging; no global write ----
+        uint32_t hc0 = 0, hc1 = 0, nl = 0;
+        bool overflow = false;
+        for (uint32_t base = 0; base < n; base += kBlock) {
+            const uint32_t r = base + threadIdx.x;
+            const bool in = r < n;
+            const uint32_t j = r >= n0 ? 1u : 0u;
+            const uint32_t idx = j ? r - n0 : r;
+            const uint64_t row = j ? s1 + idx : s0 + idx;
+            Key<W> k;
+            T c = T(0);
+            const bool valid = in;
+            if (in) {
+                k = load_key<W>(a.x, a.z, a.cap, row);
+                c = a.c[row];
+            }
+            const bool anti = valid && anticommutes_on<W>(k, g, supp);
+            const bool hole = in && !anti && !(valid && survives<W, T>(a.keep, k, c));
+            bool f[4] = {hole && j == 0, hole && j == 1, anti && j == 0, anti && j == 1};
+            uint32_t off[4], tot[4];
+            block_flag_scan<4>(f, off, tot, scan_scratch);
+            if (hole) holes[j * kInplaceHoles + (j ? hc1 + off[1] : hc0 + off[0])] = (uint16_t)idx;
+            overflow = overflow || nl + tot[2] + tot[3] > S;
+            if (anti && !overflow) {
+                const uint32_t li = nl + off[2] + off[3];
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    Lx[w * S + li] = k.x[w];
+                    Lz[w * S + li] = k.z[w];
+                }
+                Lc[li] = c;
+                Lm[li] = (uint8_t)j;
+                Li[li] = (uint16_t)idx;
+                const uint32_t fk = key_fold<W>(k);
+                Lf[li] = fk;
+                const uint32_t h = fold_mix(fk);
+                const uint32_t entry = (h << 16) | (li + 1u);
+                uint32_t sl = h >> (32u - tbits);
+                while (atomicCAS(&table[sl], 0u, entry) != 0u) sl = (sl + 1u) & tmask;
+            }
+            hc0 += tot[0];
+            hc1 += tot[1];
+            nl += tot[2] + tot[3];
+        }
+        if (overflow) {
+            if (threadIdx.x == 0) {
+                if (REDO) {
+                    atomicOr(&a.stats[2], 1ull);  // the redo launch stages the largest pair: never expected
+                } else {
+                    a.redo[atomicAdd(a.redo_count, 1ull)] = p;
+                    // a side's rows after the layer are at most the pair's
+                    atomicAdd(&a.stats[7], (unsigned long long)(2u * moved_segment_rows(n)));
+                }
+            }
+            continue;
+        }
+        __syncthreads();  // staging, table and phase A's holes complete
+        // ---- phase B: every staged row's decision, into LDS; own drops extend the holes ----
+        uint32_t nc0 = 0, nc1 = 0;
+        for (uint32_t lb = 0; lb < nl; lb += kBlock) {
+            const uint32_t i = lb + threadIdx.x;
+            const bool valid = i < nl;
+            bool keep_own = false, keep_new = false;
+            uint32_t jj = 0;
+            if (valid) {
+                Key<W> kk;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    kk.x[w] = Lx[w * S + i];
+                    kk.z[w] = Lz[w * S + i];
+                }
+                const T cc = Lc[i];
+                jj = Lm[i];
+                const Key<W> q = key_xor<W>(kk, g);
+                const uint32_t h = fold_mix(Lf[i] ^ fold_g);
+                const uint32_t tag = h << 16;
+                uint32_t sl = h >> (32u - tbits);
+                int found = -1;
+                for (;;) {
+                    const uint32_t e = table[sl];
+                    if (e == 0u) break;
+                    if ((e & 0xFFFF0000u) == tag) {
+                        const uint32_t ix = (e & 0xFFFFu) - 1u;
+                        bool eq = true;
+#pragma unroll
+                        for (int w = 0; w < W; ++w)
+                            eq = eq && (Lx[w * S + ix] == q.x[w]) && (Lz[w * S + ix] == q.z[w]);
+                        if (eq) {
+                            found = (int)ix;
+                            break;
+                        }
+                    }
+                    sl = (sl + 1u) & tmask;
+                }
+                const T a1 = partner_amplitude_on<W, T>(found >= 0 ? q : kk, g, supp, a.gphase, a.sin_t);
+                T own = a.clifford ? T(0) : cc * a.cos_t;
+                if (found >= 0) own = own + Lc[found] * a1;
+                keep_own = survives<W, T>(a.keep, kk, own);
+                Lown[i] = own;
+                if (found < 0) {
+                    const T fresh = cc * a1;
+                    keep_new = survives<W, T>(a.keep, q, fresh);
+                    Lfresh[i] = fresh;
+                }
+            }
+            const uint32_t jn = pair ? (jj ^ 1u) : jj;
+            const bool drop = valid && !keep_own;
+            bool f[4] = {drop && jj == 0, drop && jj == 1, keep_new && jn == 0, keep_new && jn == 1};
+            uint32_t off[4], tot[4];
+            block_flag_scan<4>(f, off, tot, scan_scratch);
+            if (drop) holes[jj * kInplaceHoles + (jj ? hc1 + off[1] : hc0 + off[0])] = Li[i];
+            if (valid) {
+                Lflag[i] = (uint8_t)((keep_own ? 1u : 0u) | (keep_new ? 2u : 0u));
+                Lrank[i] = (uint16_t)(jn ? nc1 + off[3] : nc0 + off[2]);
+            }
+            hc0 += tot[0];
+            hc1 += tot[1];
+            nc0 += tot[2];
+            nc1 += tot[3];
+        }
+        // rows that stay, per side: every row of the side but its holes
+        const uint32_t fin0 = n0 - hc0 + nc0, fin1 = n1 - hc1 + nc1;
+        const bool fits = hc0 <= nc0 && fin0 <= cap0 && (!pair || (hc1 <= nc1 && fin1 <= cap1));
+        __syncthreads();  // every decision and hole in LDS
+        if (!fits && !REDO) {
+            if (threadIdx.x == 0) {
+                a.redo[atomicAdd(a.redo_count, 1ull)] = p;
+                atomicAdd(&a.stats[7], (unsigned long long)(moved_segment_rows(fin0) + (pair ? moved_segment_rows(fin1) : 0u)));
+            }
+            continue;
+        }
+        if (fits) {
+            // ---- phase C in place: kept own coefficients, new rows into the holes and then after the rows ----
+            for (uint32_t i = threadIdx.x; i < nl; i += kBlock) {
+                const uint32_t fl = Lflag[i];
+                const uint32_t jj = Lm[i];
+                if (fl & 1u) a.c[(jj ? s1 : s0) + Li[i]] = Lown[i];
+                if (fl & 2u) {
+                    const uint32_t jn = pair ? (jj ^ 1u) : jj;
+                    const uint32_t rk = Lrank[i];
+                    const uint32_t hcn = jn ? hc1 : hc0;
+                    const uint32_t slot = rk < hcn ? holes[jn * kInplaceHoles + rk] : (jn ? n1 : n0) + (rk - hcn);
+                    const uint64_t pos = (jn ? s1 : s0) + slot;
+                    Key<W> q;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        q.x[w] = Lx[w * S + i] ^ g.x[w];
+                        q.z[w] = Lz[w * S + i] ^ g.z[w];
+                    }
+                    store_key<W>(a.x, a.z, a.cap, pos, q);
+                    store_coeff<T>(a.c, pos, Lfresh[i]);
+                }
+            }
+            if (threadIdx.x == 0) {
+                a.len[b0] = fin0;
+                if (pair) a.len[b1] = fin1;
+                if (a.count_totals) {
+                    atomicAdd(&a.stats[0], (unsigned long long)(fin0 + fin1));
+                    atomicMax(&a.stats[1], (unsigned long long)max(fin0, fin1));
+                }
+            }
+            continue;
+        }
+        // ---- phase C moved: the pair's rows that stay, written whole to fresh segments ----
+        const uint32_t seg0 = moved_segment_rows(fin0);
+        const uint32_t seg1 = pair ? moved_segment_rows(fin1) : 0u;
+        if (threadIdx.x == 0) {
+            const uint64_t base = a.tail_base + atomicAdd(&a.stats[5], (unsigned long long)(seg0 + seg1));
+            if (base + seg0 + seg1 > a.tail_limit) {
+                atomicOr(&a.stats[2], 2ull);  // the host's bound on moved rows was wrong: nothing is written
+                reloc_base = ~0ull;
+            } else {
+                reloc_base = base;
+            }
+        }
+        __syncthreads();
+        const uint64_t nb0 = reloc_base;
+        if (nb0 == ~0ull) continue;
+        const uint64_t nb1 = nb0 + seg0;
+        // commuting rows that stay, in the order of the pair's rows (the same reads as phase A: nothing was written)
+        uint32_t cur0 = 0, cur1 = 0;
+        for (uint32_t base = 0; base < n; base += kBlock) {
+            const uint32_t r = base + threadIdx.x;
+            const bool in = r < n;
+            const uint32_t j = r >= n0 ? 1u : 0u;
+            const uint64_t row = j ? s1 + (r - n0) : s0 + r;
+            Key<W> k;
+            T c = T(0);
+            const bool valid = in;
+            if (in) {
+                k = load_key<W>(a.x, a.z, a.cap, row);
+                c = a.c[row];
+            }
+            const bool stay = valid && !anticommutes_on<W>(k, g, supp) && survives<W, T>(a.keep, k, c);
+            bool f[2] = {stay && j == 0, stay && j == 1};
+            uint32_t off[2], tot[2];
+            block_flag_scan<2>(f, off, tot, scan_scratch);
+            if (stay) {
+                const uint64_t pos = j ? nb1 + cur1 + off[1] : nb0 + cur0 + off[0];
+                store_key<W>(a.x, a.z, a.cap, pos, k);
+                store_coeff<T>(a.c, pos, c);
+            }
+            cur0 += tot[0];
+            cur1 += tot[1];
+        }
+        // then the kept own rows, then the new rows (their ranks from phase B)
+        for (uint32_t lb = 0; lb < nl; lb += kBlock) {
+            const uint32_t i = lb + threadIdx.x;
+            const bool valid = i < nl;
+            const uint32_t fl = valid ? Lflag[i] : 0u;
+            const uint32_t jj = valid ? Lm[i] : 0u;
+            bool f[2] = {(fl & 1u) && jj == 0, (fl & 1u) && jj == 1};
+            uint32_t off[2], tot[2];
+            block_flag_scan<2>(f, off, tot, scan_scratch);
+            if (fl & 1u) {
+                const uint64_t pos = jj ? nb1 + cur1 + off[1] : nb0 + cur0 + off[0];
+                Key<W> kk;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    kk.x[w] = Lx[w * S + i];
+                    kk.z[w] = Lz[w * S + i];
+                }
+                store_key<W>(a.x, a.z, a.cap, pos, kk);
+                store_coeff<T>(a.c, pos, Lown[i]);
+            }
+            cur0 += tot[0];
+            cur1 += tot[1];
+        }
+        for (uint32_t i = threadIdx.x; i < nl; i += kBlock) {
+            if (!(Lflag[i] & 2u)) continue;
+            const uint32_t jn = pair ? (Lm[i] ^ 1u) : Lm[i];
+            const uint64_t pos = jn ? nb1 + cur1 + Lrank[i] : nb0 + cur0 + Lrank[i];
+            Key<W> q;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                q.x[w] = Lx[w * S + i] ^ g.x[w];
+                q.z[w] = Lz[w * S + i] ^ g.z[w];
+            }
+            store_key<W>(a.x, a.z, a.cap, pos, q);
+            store_coeff<T>(a.c, pos, Lfresh[i]);
+        }
+        if (threadIdx.x == 0) {
+            a.start[b0] = nb0;
+            a.len[b0] = fin0;
+            a.bcap[b0] = seg0;
+            if (pair) {
+                a.start[b1] = nb1;
+                a.len[b1] = fin1;
+                a.bcap[b1] = seg1;
+            }
+            atomicAdd(&a.stats[6], 1ull);
+            if (a.count_totals) {
+                atomicAdd(&a.stats[0], (unsigned long long)(fin0 + fin1));
+                atomicMax(&a.stats[1], (unsigned long long)max(fin0, fin1));
+            }
+        }
+    }
+}
+
 // Many sums' layers in one launch: workgroup w joins pair w - pair_start[i] of item i, where item i is the one
 // whose range [pair_start[i], pair_start[i + 1]) holds w.  Every item stages its largest pair (no redo).
 template <int W, typename T, int BLOCK>
@@ -816,6 +1151,10 @@ __global__ void __launch_bounds__(kBlock) k_refine(const RefineArgs<W, T> a) {
         a.olen[b] = cur0;
         a.ostart[b + a.num_buckets] = s + cur0;
         a.olen[b + a.num_buckets] = cur1;
+        if (a.obcap) {  // the segment's room after its rows goes with the second half
+            a.obcap[b] = cur0;
+            a.obcap[b + a.num_buckets] = a.bcap[b] - cur0;
+        }
     }
 }
 
@@ -910,8 +1249,9 @@ uint32_t offsets_tiles(uint32_t out_n) { return (out_n + kScanTile - 1) / kScanT
 hipError_t launch_bucket_offsets(hipStream_t st, const uint32_t* len, uint32_t num_buckets, uint32_t mode,
                                  uint32_t delta, uint32_t coarse_buckets, uint32_t fold, uint64_t* out_start,
                                  unsigned long long* total, uint64_t* tile_sums, unsigned long long* clear,
-                                 uint32_t clear_n) {
+                                 uint32_t clear_n, uint32_t* out_cap) {
     OffsetsArgs a{};
+    a.out_cap = mode == 0 ? nullptr : out_cap;
     a.clear = clear;
     a.clear_n = clear ? clear_n : 0u;
     a.len = len;
@@ -996,6 +1336,32 @@ hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_
     return hipGetLastError();
 }
 
+// One workgroup per pair; the redo launch, over the queued pairs, is a grid of as many as stay resident.
+template <int W, typename T>
+hipError_t launch_rotation_inplace(hipStream_t st, const InplaceArgs<W, T>& a) {
+    if (a.num_cosets == 0) return hipSuccess;
+    if (a.scap > kPairRowsMax || a.tcap_bits > 11) return hipErrorInvalidValue;
+    const size_t lds = inplace_lds_bytes<W, T>(a.scap, a.tcap_bits);
+    if (!a.redo_mode) {
+        hipLaunchKernelGGL((k_rotation_inplace<W, T, false>), dim3(a.num_cosets), dim3(kBlock), lds, st, a);
+        return hipGetLastError();
+    }
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rotation_inplace<W, T, true>, kBlock, lds) !=
+            hipSuccess ||
+        per <= 0)
+        per = 1;
+    hipLaunchKernelGGL((k_rotation_inplace<W, T, true>), dim3(cus * per), dim3(kBlock), lds, st, a);
+    return hipGetLastError();
+}
+
 template <int W, typename T>
 hipError_t launch_join_batch(hipStream_t st, const JoinArgs<W, T>* items, const uint32_t* pair_start,
                              uint32_t num_items, uint32_t total_pairs, uint32_t scap_max, uint32_t tcap_bits_max,
@@ -1066,6 +1432,8 @@ hipError_t launch_export(hipStream_t st, const ExportArgs<W, T>& a, uint32_t num
     template hipError_t launch_join_batch<W, T>(hipStream_t, const JoinArgs<W, T>*, const uint32_t*, uint32_t,     \
                                                 uint32_t, uint32_t, uint32_t, unsigned long long*);             \
     template hipError_t launch_rescale<W, T>(hipStream_t, const RescaleArgs<W, T>&, uint32_t);          \
+    template size_t inplace_lds_bytes<W, T>(uint32_t, uint32_t);                                        \
+    template hipError_t launch_rotation_inplace<W, T>(hipStream_t, const InplaceArgs<W, T>&);           \
     template hipError_t launch_perm_walk<W, T>(hipStream_t, const PermWalkArgs<W, T>&, uint32_t);       \
     template hipError_t launch_perm_scatter<W, T>(hipStream_t, const PermScatterArgs<W, T>&, uint32_t); \
     template hipError_t launch_refine<W, T>(hipStream_t, const RefineArgs<W, T>&);                      \

@@ -287,6 +287,10 @@ class GpuPauliSum:
         _native.check(_native.lib().pp_sum_clone(self._h, ctypes.byref(out._h)), self.ctx.handle)
         return out
 
+    def copy_from(self, src: "GpuPauliSum") -> None:
+        """Take ``src``'s terms and layout into this sum's own device memory (pp_sum_copy; asynchronous)."""
+        _native.check(_native.lib().pp_sum_copy(self._h, src._h), self.ctx.handle)
+
     def bucket_bits(self) -> tuple[int, int]:
         r = ctypes.c_int32()
         d = ctypes.c_int32()

@@ -44,10 +44,11 @@ class Options(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    """pp_tuning: the device path's sizes (bucket_target, stage_rows, groups_per_cu, join_block)."""
+    """pp_tuning: the device path's sizes (bucket_target, stage_rows, groups_per_cu, join_block, join_in_place)."""
 
     _fields_ = [("bucket_target", ctypes.c_uint32), ("stage_rows", ctypes.c_uint32),
-                ("groups_per_cu", ctypes.c_uint32), ("join_block", ctypes.c_uint32)]
+                ("groups_per_cu", ctypes.c_uint32), ("join_block", ctypes.c_uint32),
+                ("join_in_place", ctypes.c_uint32)]
 
 
 ONE_PAIR_PER_GROUP = 0xFFFFFFFF
@@ -116,6 +117,7 @@ def lib() -> ctypes.CDLL:
     L.pp_context_stream.argtypes = [vp, pvp]
     L.pp_sum_upload.argtypes = [vp, u32, u32, ctypes.c_int, u64, vp, vp, vp, u64, pvp]
     L.pp_sum_clone.argtypes = [vp, pvp]
+    L.pp_sum_copy.argtypes = [vp, vp]
     L.pp_sum_free.argtypes = [vp]
     L.pp_sum_len.argtypes = [vp, ctypes.POINTER(u64)]
     L.pp_sum_download.argtypes = [vp, vp, vp, vp]
@@ -161,7 +163,8 @@ def lib() -> ctypes.CDLL:
                  "pp_batch_propagate",
                  "pp_expectation_product", "pp_profile_enable", "pp_profile_read", "pp_profile_read_score",
                  "pp_prepare_channel", "pp_apply_prepared", "pp_apply_run", "pp_finalize_layer", "pp_settle",
-                 "pp_sum_refine", "pp_sum_coarsen_to", "pp_stream_probe", "pp_apply_rotation_to_noalloc"):
+                 "pp_sum_refine", "pp_sum_coarsen_to", "pp_stream_probe", "pp_apply_rotation_to_noalloc",
+                 "pp_sum_copy"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -233,8 +236,10 @@ class Context:
         return t
 
     def set_tuning(self, bucket_target: int, stage_rows: int = 0, groups_per_cu: int = ONE_PAIR_PER_GROUP,
-                   join_block: int = 256) -> None:
-        t = Tuning(bucket_target, stage_rows, groups_per_cu, join_block)
+                   join_block: int = 256, join_in_place: int | None = None) -> None:
+        if join_in_place is None:
+            join_in_place = self.get_tuning().join_in_place
+        t = Tuning(bucket_target, stage_rows, groups_per_cu, join_block, join_in_place)
         check(lib().pp_set_tuning(self.handle, ctypes.byref(t)), self.handle)
 
     def capture(self) -> "GraphCapture":

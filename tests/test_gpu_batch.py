@@ -79,3 +79,17 @@ def test_batch_of_wide_sums_forward():
             c.rzz(float(rng.uniform(0, 2 * math.pi)), q, q + 1).rx(float(rng.uniform(0, 2 * math.pi)), int(rng.integers(0, n)))
         inst.append((n, c, (x, z, rng.standard_normal(m)), 1e-4))
     check_batch(inst, [po.coeff_policy(eps=1e-4)] * 3, [truncation.coeff(1e-4)] * 3, direction="forward")
+
+
+@pytest.mark.timeout(900)
+def test_batched_workload_at_its_own_parameters():
+    """The batched workload as the benchmark draws it: 64 instances, 16..64 qubits, depths 5..30, thresholds
+    log-uniform in [1e-6, 1e-3], Heisenberg, f64.  Uncapped its deepest instances grow past 10^9 terms, and the
+    benchmark caps each at TopN 2^20; here the cap is 2^12 so that the oracle (whose TopN sorts every term after
+    every run) finishes all 64 solo runs in well under a minute.  Every instance equals its solo oracle run."""
+    announce("batch of the 64 benchmark instances (depths 5..30, thresholds 1e-6..1e-3), TopN 2^12")
+    inst = circuits.batched_instances(seed=0xC0FFEE, count=64)
+    cap = 1 << 12
+    pol_o = [po.coeff_policy(eps=eps, topn=cap) for (_n, _c, _o, eps) in inst]
+    pol_p = [truncation.coeff(eps) & truncation.topn(cap) for (_n, _c, _o, eps) in inst]
+    check_batch(inst, pol_o, pol_p)

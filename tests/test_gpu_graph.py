@@ -4,6 +4,7 @@ the eager layers.  Each case prints its name before it touches the GPU."""
 
 from __future__ import annotations
 
+import math
 import sys
 
 import numpy as np
@@ -95,15 +96,16 @@ def test_synchronizing_calls_are_refused_during_capture():
 
 
 def test_replay_is_refused_once_the_captured_buffers_changed():
-    """A replay writes through the buffers the capture baked in: once the output took a layer in place (its current
-    column set flipped) the launch is refused on the host, before anything is queued, and a sum a live graph uses
-    cannot be freed.  The refusal needs no kernel: the check runs before the launch."""
-    announce("graph replay refused after an in-place layer on its output")
+    """A replay writes through the buffers the capture baked in.  After a layer in place on its output (same
+    columns, same bucket arrays) a replay is accepted and rewrites the output whole; once the output's current
+    column set flipped (a quarter turn runs out of place) the launch is refused on the host, before anything is
+    queued, and a sum a live graph uses cannot be freed.  The refusal needs no kernel: the check runs first."""
+    announce("graph replay refused after an out-of-place layer on its output")
     from paulistrings_amd import _native
 
     rng = np.random.default_rng(43)
     nq = 60
-    n = 12000  # its layer stays below the next device refinement (64 buckets of 256): the layer flips once
+    n = 12000  # its layers stay below the next device refinement (64 buckets of 256)
     x, z = random_keys(rng, n, nq, 0.1)
     c = rng.standard_normal(n)
     src = PauliSum(nq, x, z, c).to_device(0)
@@ -120,11 +122,15 @@ def test_replay_is_refused_once_the_captured_buffers_changed():
     assert all(np.array_equal(a, b) for a, b in zip(got, want))
     assert _native.lib().pp_sum_free(dst._h) == 5  # PP_ERR_INVALID_ARGUMENT: the graph writes it
     bits = dst.bucket_bits()[1]
-    dst.apply_rotation(Circuit(nq).rx(0.3, 7).channels(), policy=policy)  # in place: the current set flips
+    dst.apply_rotation(Circuit(nq).rx(0.3, 7).channels(), policy=policy)  # in place: the buffers stay
+    cap.graph.launch()
+    got = terms(dst)
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))  # the replay rewrote the output whole
+    dst.apply_rotation(Circuit(nq).rzz(-math.pi / 2, 8, 9).channels(), policy=policy)  # out of place: flips
     assert dst.bucket_bits()[1] == bits  # no refinement in between (a refinement flips it back)
     with pytest.raises(Exception, match="changed since the capture"):
         cap.graph.launch()
-    after = terms(dst)  # the in-place layer's result, untouched by the refused replay
+    after = terms(dst)  # the quarter turn's result, untouched by the refused replay
     assert len(after[2]) > 0
     cap.graph.free()
     dst.free()

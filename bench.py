@@ -582,6 +582,9 @@ def main() -> int:
     ap.add_argument("--cpu-log2n", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--main-only", action="store_true",
+                    help="only the in-place line (no out-of-place comparison, no float32 line, no STREAM probe): "
+                         "for profiler runs")
     ap.add_argument("--workload", choices=["saturated", "learned-score", "batched", "tfim6x6", "kicked-ising"],
                     default="saturated",
                     help="saturated: the headline line (default); learned-score: the learned pass's MLP scoring on "
@@ -628,13 +631,15 @@ def main() -> int:
     x, z, c = saturated_keys(1 << args.log2n, SEED, args.dtype)
     log(f"{len(c)} distinct keys; warmup {args.warmup}, timed {args.steps} steps of {len(channels)} layers")
     r = gpu_measure_in_place(x, z, c, channels, args.steps, args.warmup)
-    log("the same layers out of place (into a second sum, queued back to back)")
-    oop = summarize(gpu_measure(x, z, c, channels, args.steps, args.warmup), args.dtype, args.steps)
-    log(f"out of place: {oop['value']:.3e} merged terms/s, join kernel {oop['achieved']:.0f} GB/s")
-    stream = stream_probe()
-    log("STREAM probe (GB/s): " + ", ".join(f"{k} {v:.0f}" for k, v in stream.items()))
-    other = None
-    if args.dtype == "f64":  # the same keys with the coefficients rounded to float32: the layer in float32
+    oop = other = None
+    stream = {}
+    if not args.main_only:
+        log("the same layers out of place (into a second sum, queued back to back)")
+        oop = summarize(gpu_measure(x, z, c, channels, args.steps, args.warmup), args.dtype, args.steps)
+        log(f"out of place: {oop['value']:.3e} merged terms/s, join kernel {oop['achieved']:.0f} GB/s")
+        stream = stream_probe()
+        log("STREAM probe (GB/s): " + ", ".join(f"{k} {v:.0f}" for k, v in stream.items()))
+    if args.dtype == "f64" and not args.main_only:  # the same keys, coefficients rounded to float32
         log(f"float32 coefficients: timed {args.steps} steps of {len(channels)} layers")
         other = summarize(gpu_measure_in_place(x, z, c.astype(np.float32), channels, args.steps, args.warmup),
                           "f32", args.steps)

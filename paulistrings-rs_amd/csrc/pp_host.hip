@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/pp_hip.h"
+#include "pp_class.h"
 #include "pp_kernels.h"
 #include "pp_learned.h"
 #include "pp_membench.h"
@@ -277,6 +278,9 @@ struct pp_context {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> score_events;  // the learned pass's score kernel
     uint64_t prof_scored = 0;
     unsigned long long* d_prof = nullptr;  // [0]: output rows of the profiled joins
+    void* stage[2] = {nullptr, nullptr};   // pinned host chunks of the bulk transfers, allocated on first use
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    void* class_pinned = nullptr;  // a class pass's program on its way up and its counters on their way down
 };
 
 struct pp_graph {
@@ -313,6 +317,17 @@ enum ScratchSlot {
     kScrRunCounts,   // its per-bucket destination counts
     kScrDeaths,      // its per-step drops
     kScrSteps,       // its steps
+    kScrClassCount,  // a class pass's counters, outputs per bucket and touched rows per class bucket
+    kScrClassStart,  // the class buckets' scan
+    kScrClassKeep,   // its untouched rows kept per bucket
+    kScrClassLayers, // its program
+    kScrClassSX,     // its touched rows in class-bucket order
+    kScrClassSZ,
+    kScrClassSC,
+    kScrClassOX,     // its outputs
+    kScrClassOZ,
+    kScrClassOC,
+    kScrClassOB,     // their buckets
     kScratchSlots
 };
 
@@ -532,6 +547,72 @@ pp_status count_layout(pp_sum* s, int which, uint32_t num_buckets) {
     if ((st = read_stats(s)) != PP_OK) return st;
     s->len = s->h_stats[0];
     s->max_bucket = (uint32_t)s->h_stats[1];
+    return PP_OK;
+}
+
+// ---- pinned, chunked host transfers (the reference's staging of uploads and downloads, engine/gpu/staging.rs) -------
+
+constexpr size_t kStageChunk = (size_t)32 << 20;  // bytes per pinned chunk; two alternate
+
+pp_status ensure_staging(pp_context* ctx) {
+    if (ctx->stage[0]) return PP_OK;
+    for (int k = 0; k < 2; ++k) {
+        if (hipHostMalloc(&ctx->stage[k], kStageChunk, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->stage[k] = nullptr;
+            return fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned staging allocation");
+        }
+        PP_HIP(ctx, hipEventCreateWithFlags(&ctx->stage_ev[k], hipEventDisableTiming));
+        PP_HIP(ctx, hipEventRecord(ctx->stage_ev[k], ctx->stream));  // recorded once, so a first wait is defined
+    }
+    return PP_OK;
+}
+
+// The host's copy into or out of a pinned chunk, on up to 16 threads for a full chunk.
+void stage_memcpy(void* dst, const void* src, size_t n) {
+    const unsigned T = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(host_threads(n >> 3), 8));
+    host_parallel(T, [&](unsigned t, unsigned TT) {
+        uint64_t lo, hi;
+        part_range(n, t, TT, lo, hi);
+        std::memcpy((char*)dst + lo, (const char*)src + lo, hi - lo);
+    });
+}
+
+// host -> device through the two pinned chunks: the host fills one while the other's copy runs.  Queued on the
+// context's stream; the caller synchronizes before it reads the result.
+pp_status staged_h2d(pp_context* ctx, void* dev, const void* host, size_t bytes) {
+    pp_status st;
+    if ((st = ensure_staging(ctx)) != PP_OK) return st;
+    int k = 0;
+    for (size_t off = 0; off < bytes; off += kStageChunk, k ^= 1) {
+        const size_t n = std::min(kStageChunk, bytes - off);
+        PP_HIP(ctx, hipEventSynchronize(ctx->stage_ev[k]));  // the chunk's previous copy has drained
+        stage_memcpy(ctx->stage[k], (const char*)host + off, n);
+        PP_HIP(ctx, hipMemcpyAsync((char*)dev + off, ctx->stage[k], n, hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(ctx, hipEventRecord(ctx->stage_ev[k], ctx->stream));
+    }
+    return PP_OK;
+}
+
+// device -> host through the two pinned chunks: chunk i + 1's copy runs while the host drains chunk i.  Returns once
+// every byte is in `host`.
+pp_status staged_d2h(pp_context* ctx, void* host, const void* dev, size_t bytes) {
+    pp_status st;
+    if ((st = ensure_staging(ctx)) != PP_OK) return st;
+    const size_t chunks = (bytes + kStageChunk - 1) / kStageChunk;
+    auto queue = [&](size_t i) -> pp_status {
+        const size_t off = i * kStageChunk, n = std::min(kStageChunk, bytes - off);
+        PP_HIP(ctx, hipMemcpyAsync(ctx->stage[i & 1], (const char*)dev + off, n, hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(ctx, hipEventRecord(ctx->stage_ev[i & 1], ctx->stream));
+        return PP_OK;
+    };
+    if (chunks && (st = queue(0)) != PP_OK) return st;
+    for (size_t i = 0; i < chunks; ++i) {
+        if (i + 1 < chunks && (st = queue(i + 1)) != PP_OK) return st;  // its chunk was drained one step ago
+        PP_HIP(ctx, hipEventSynchronize(ctx->stage_ev[i & 1]));
+        const size_t off = i * kStageChunk;
+        stage_memcpy((char*)host + off, ctx->stage[i & 1], std::min(kStageChunk, bytes - off));
+    }
     return PP_OK;
 }
 
@@ -1114,8 +1195,17 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, cons
 // Whether a join layer runs in place (k_rotation_inplace) rather than into the spare columns: the layout's segment
 // rooms are known, no TopN selection is pending (every row it drops would be a hole to fill), the layer is not a
 // Clifford arm (every anticommuting row would be one) and earlier moves have not left more dead rows than live ones.
+// Below 2^18 rows the layer is launch-bound and the out-of-place join, one synchronization, is the faster
+// (PP_IN_PLACE_MIN_ROWS overrides the bound, for tests of the in-place path on small sums).
+uint64_t in_place_min_rows() {
+    const char* e = std::getenv("PP_IN_PLACE_MIN_ROWS");
+    return e ? std::strtoull(e, nullptr, 10) : (1ull << 18);
+}
+
 bool join_in_place(const pp_sum* s, const Prepared& p) {
-    if (!s->ctx->tuning.join_in_place || !s->has_room || s->pending_topn || p.clifford || s->len == 0) return false;
+    if (!s->ctx->tuning.join_in_place || !s->has_room || s->pending_topn || p.clifford || s->len == 0 ||
+        s->len < in_place_min_rows())
+        return false;
     const uint64_t B = 1ull << s->device_bits;
     return s->tail <= 3 * s->len + 4ull * kSegAlign * B;
 }
@@ -1259,11 +1349,18 @@ pp_status layer_in_place(pp_sum* s, const Prepared& p, const KeepProgram& keep) 
     return PP_OK;
 }
 
+pp_status class_pass(pp_sum* s, const Prepared* layers, uint32_t n, const KeepProgram& keep, uint64_t* len_after,
+                     bool* done);  // below
+
 pp_status device_layer(pp_sum* s, const Prepared& p, const KeepProgram& keep) {
     pp_context* ctx = s->ctx;
     pp_status st;
-    if (!p.key_preserving && !p.rotation)
-        return fail(ctx, PP_ERR_UNSUPPORTED, "a local table of this shape has no device layer yet");
+    if (!p.key_preserving && !p.rotation) {  // any other local table: a one-layer class pass
+        bool done = false;
+        if ((st = class_pass(s, &p, 1, keep, nullptr, &done)) != PP_OK) return st;
+        if (!done) return fail(ctx, PP_ERR_DRIVER, "class pass: a class bucket outgrew the largest LDS set");
+        return PP_OK;
+    }
     if (p.key_preserving && (st = materialize_topn(s)) != PP_OK) return st;
     if ((st = settle_device_bits(s)) != PP_OK) return st;
     const uint32_t B = 1u << s->device_bits;
@@ -1398,6 +1495,262 @@ pp_status permutation_run_device(pp_sum* s, const Prepared* steps, uint32_t n_st
     return PP_OK;
 }
 
+// ---- the class pass: a program of local tables in one pass (pp_class.hip) ----------------------------------------
+
+constexpr uint32_t kClassMaxQubits = 16;   // a program's qubits together (RR_MAX_QUBITS of the reference's run)
+constexpr uint32_t kClassMinRunLayers = 3; // rotation runs at least this long take the class pass
+
+// The device form of a local table: its slots, its entries (a Clifford rotation table without the identity residue
+// the pair join drops), and the patterns it changes.  A pattern is left alone when the identity entry keeps it with
+// amplitude exactly 1, no other entry emits from it and none emits into it.
+ClassLayer class_layer(const Prepared& p) {
+    ClassLayer L{};
+    const pp_prepared& t = p.table;
+    const uint32_t q0 = t.k ? t.qubits[0] : 0, q1 = t.k ? t.qubits[t.k - 1] : 0;
+    L.word[0] = q0 / 64;
+    L.bit[0] = q0 % 64;
+    L.word[1] = q1 / 64;
+    L.bit[1] = q1 % 64;
+    L.mask = (1u << (2 * t.k)) - 1u;
+    const uint32_t dim = 1u << (2 * t.k);
+    L.n_entries = t.n_deltas;
+    int identity = -1;
+    for (uint32_t e = 0; e < t.n_deltas; ++e) {
+        L.delta[e] = t.deltas[e].local_delta;
+        if (L.delta[e] == 0) identity = (int)e;
+        for (uint32_t q = 0; q < 16; ++q) L.amp[e][q] = q < dim ? t.deltas[e].amplitude[q] : 0.0;
+    }
+    if (p.rotation_shaped && p.clifford && identity == 0 && t.n_deltas == 2)
+        for (uint32_t q = 0; q < dim; ++q)
+            if (L.amp[1][q] != 0.0) L.amp[0][q] = 0.0;
+    for (uint32_t q = 0; q < dim; ++q) {
+        bool still = identity >= 0 && L.amp[identity][q] == 1.0;
+        for (uint32_t e = 0; e < t.n_deltas && still; ++e)
+            if ((int)e != identity) still = L.amp[e][q] == 0.0 && L.amp[e][q ^ L.delta[e]] == 0.0;
+        if (!still) L.active |= 1u << q;
+    }
+    return L;
+}
+
+// Whether layers[0..n) fit one class pass: local tables whose qubits number at most kClassMaxQubits.
+bool class_program_fits(const Prepared* layers, uint32_t n) {
+    if (n == 0 || n > kClassMaxLayers) return false;
+    std::vector<uint32_t> qs;
+    for (uint32_t l = 0; l < n; ++l) {
+        if (!layers[l].local) return false;
+        for (uint32_t j = 0; j < layers[l].table.k; ++j) qs.push_back(layers[l].table.qubits[j]);
+    }
+    std::sort(qs.begin(), qs.end());
+    qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
+    return qs.size() <= kClassMaxQubits;
+}
+
+// Whether pp_propagate and pp_apply_run give rotation runs to the class pass (default) or apply them layer by layer
+// (PP_ROTATION_RUNS=layers, for comparison: both give the same keys, coefficients and per-layer lengths).
+bool class_rotation_runs() {
+    const char* e = std::getenv("PP_ROTATION_RUNS");
+    return !(e && std::strcmp(e, "layers") == 0);
+}
+
+// layers[0..n) in one class pass, the sum's length after each into len_after (optional).  *done = false, the sum as it
+// was, when a class bucket outgrew the LDS sets at every class bucketing tried: the caller applies the layers one by
+// one.  Two synchronizations: the counters after the apply, the layout's length at the end.
+pp_status class_pass(pp_sum* s, const Prepared* layers, uint32_t n, const KeepProgram& keep, uint64_t* len_after,
+                     bool* done) {
+    pp_context* ctx = s->ctx;
+    pp_status st;
+    *done = false;
+    if (!class_program_fits(layers, n)) return PP_OK;
+    if ((st = materialize_topn(s)) != PP_OK || (st = settle_device_bits(s)) != PP_OK) return st;
+    const uint64_t len_in = s->len;
+    if (len_in == 0) {
+        for (uint32_t l = 0; l < n && len_after; ++l) len_after[l] = 0;
+        *done = true;
+        return PP_OK;
+    }
+    const uint32_t B = 1u << s->device_bits;
+    // the program and the counters pass through one pinned block of the context (the previous pass has drained it:
+    // every pass ends with a synchronization)
+    constexpr size_t kPinnedStats = kClassMaxLayers * sizeof(ClassLayer);
+    if (!ctx->class_pinned &&
+        hipHostMalloc(&ctx->class_pinned, kPinnedStats + kClassStatSlots * 8, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->class_pinned = nullptr;
+        return fail(ctx, PP_ERR_OUT_OF_MEMORY, "class pass: pinned host block");
+    }
+    ClassLayer* prog = static_cast<ClassLayer*>(ctx->class_pinned);
+    unsigned long long* h = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->class_pinned) + kPinnedStats);
+    uint64_t qmask[4] = {};
+    for (uint32_t l = 0; l < n; ++l) {
+        prog[l] = class_layer(layers[l]);
+        for (uint32_t j = 0; j < layers[l].table.k; ++j)
+            qmask[layers[l].table.qubits[j] / 64] |= 1ull << (layers[l].table.qubits[j] % 64);
+    }
+    ClassLayer* d_prog = nullptr;
+    uint32_t* d_ukeep = nullptr;
+    if ((st = slot_scratch(s, kScrClassLayers, n * sizeof(ClassLayer), &d_prog)) != PP_OK ||
+        (st = slot_scratch(s, kScrClassKeep, (size_t)B * 4, &d_ukeep)) != PP_OK)
+        return st;
+    PP_HIP(ctx, hipMemcpyAsync(d_prog, prog, n * sizeof(ClassLayer), hipMemcpyHostToDevice, ctx->stream));
+    const size_t cb = s->coeff_bytes();
+    uint64_t *sx = nullptr, *sz = nullptr;
+    void* sc = nullptr;
+    if ((st = slot_scratch(s, kScrClassSX, (size_t)s->W * len_in * 8, &sx)) != PP_OK ||
+        (st = slot_scratch(s, kScrClassSZ, (size_t)s->W * len_in * 8, &sz)) != PP_OK ||
+        (st = slot_scratch(s, kScrClassSC, len_in * cb, &sc)) != PP_OK)
+        return st;
+    // class buckets of about 64 rows of the input, finer on an overflow; a larger LDS set last
+    int cbits = 0;
+    while (cbits < 22 && ((uint64_t)64 << cbits) < len_in) ++cbits;
+    uint32_t lcap = 512, tbits = 10;
+    uint64_t ocap = std::max<uint64_t>(2 * len_in, 4096);
+    uint64_t *ox = nullptr, *oz = nullptr;
+    void* oc = nullptr;
+    uint32_t* ob = nullptr;
+    unsigned long long* d_cst = nullptr;
+    uint32_t* d_ocount = nullptr;
+    int attempt = 0;
+    for (;;) {
+        const uint32_t C = 1u << cbits;
+        // the counters, the outputs per bucket and the rows per class bucket: one block, one clear
+        unsigned char* d_counts = nullptr;
+        uint64_t* d_cstart = nullptr;
+        const size_t count_bytes = kClassStatSlots * 8 + (size_t)B * 4 + (size_t)C * 4;
+        if ((st = slot_scratch(s, kScrClassCount, count_bytes, &d_counts)) != PP_OK ||
+            (st = slot_scratch(s, kScrClassStart, (size_t)C * 8, &d_cstart)) != PP_OK ||
+            (st = slot_scratch(s, kScrClassOX, (size_t)s->W * ocap * 8, &ox)) != PP_OK ||
+            (st = slot_scratch(s, kScrClassOZ, (size_t)s->W * ocap * 8, &oz)) != PP_OK ||
+            (st = slot_scratch(s, kScrClassOC, ocap * cb, &oc)) != PP_OK ||
+            (st = slot_scratch(s, kScrClassOB, ocap * 4, &ob)) != PP_OK || (st = ensure_tiles(s, C)) != PP_OK)
+            return st;
+        d_cst = reinterpret_cast<unsigned long long*>(d_counts);
+        d_ocount = reinterpret_cast<uint32_t*>(d_counts + kClassStatSlots * 8);
+        uint32_t* const d_ccount = d_ocount + B;
+        PP_HIP(ctx, hipMemsetAsync(d_counts, 0, count_bytes, ctx->stream));
+        hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
+            constexpr int W = decltype(Wc)::value;
+            using T = decltype(Tv);
+            ClassArgs<W, T> a{};
+            a.x = s->col[s->cur].x;
+            a.z = s->col[s->cur].z;
+            a.c = (const T*)s->col[s->cur].c;
+            a.cap = s->col[s->cur].cap;
+            a.start = s->bk[s->cur].start;
+            a.len = s->bk[s->cur].len;
+            a.num_buckets = B;
+            a.bits = s->device_bits;
+            a.rows = s->d_rows;
+            a.cbits = cbits;
+            a.num_classes = C;
+            for (int w = 0; w < W; ++w) a.qx[w] = a.qz[w] = qmask[w];
+            a.layers = d_prog;
+            a.n_layers = n;
+            a.keep = keep;
+            a.ccount = d_ccount;
+            a.cstart = d_cstart;
+            a.ukeep = d_ukeep;
+            a.sx = sx;
+            a.sz = sz;
+            a.sc = (T*)sc;
+            a.scap = len_in;
+            a.ox = ox;
+            a.oz = oz;
+            a.oc = (T*)oc;
+            a.obucket = ob;
+            a.ocap = ocap;
+            a.ocount = d_ocount;
+            a.stats = d_cst;
+            a.lcap = lcap;
+            a.tbits = tbits;
+            hipError_t r = launch_class_route<W, T>(ctx->stream, a);
+            if (r == hipSuccess)
+                r = launch_bucket_offsets(ctx->stream, d_ccount, C, 0, 0, 0, 0, d_cstart, d_cst + kClassStaged,
+                                          s->d_tiles);
+            if (r == hipSuccess) r = launch_class_gather<W, T>(ctx->stream, a);
+            if (r == hipSuccess) r = launch_class_apply<W, T>(ctx->stream, a);
+            return r;
+        });
+        PP_HIP(ctx, e);
+        PP_HIP(ctx, hipMemcpyAsync(h, d_cst, kClassStatSlots * 8, hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const unsigned long long flags = h[kClassFlags];
+        if (flags == 0) break;
+        if ((flags & 3ull) == 0) {  // only the output staging ran out: its exact need is known now
+            ocap = h[kClassOutRows];
+        } else if (attempt == 0) {
+            cbits = std::min(cbits + 2, 24);
+        } else if (attempt == 1) {
+            cbits = std::min(cbits + 1, 24);
+            lcap = 1024;
+            tbits = 11;
+        } else {
+            return PP_OK;  // a class that outgrows the largest set: layer by layer
+        }
+        if (flags & 3ull) ++attempt;
+    }
+    // the output layout in the spare columns: each bucket's untouched rows, then its class outputs
+    const uint64_t out_rows = h[kClassOutRows], total = h[kClassUntouched] + out_rows;
+    const int in = s->cur, out = 1 - s->cur;
+    if ((st = ensure_columns(s, out, total + (uint64_t)kSegAlign * B)) != PP_OK) return st;
+    if ((st = ensure_buckets(s, out, B)) != PP_OK) return st;
+    hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
+        constexpr int W = decltype(Wc)::value;
+        using T = decltype(Tv);
+        ClassArgs<W, T> a{};
+        a.num_buckets = B;
+        a.ukeep = d_ukeep;
+        a.ocount = d_ocount;
+        a.plen = s->bk[out].len;
+        return launch_class_lens<W, T>(ctx->stream, a);
+    });
+    PP_HIP(ctx, e);
+    if ((st = offsets(s, s->bk[out].len, B, 3, 0, 0, 0, s->bk[out].start, false, s->bk[out].rows)) != PP_OK) return st;
+    e = dispatch(s, [&](auto Wc, auto Tv) {
+        constexpr int W = decltype(Wc)::value;
+        using T = decltype(Tv);
+        ClassArgs<W, T> a{};
+        a.x = s->col[in].x;
+        a.z = s->col[in].z;
+        a.c = (const T*)s->col[in].c;
+        a.cap = s->col[in].cap;
+        a.start = s->bk[in].start;
+        a.len = s->bk[in].len;
+        a.num_buckets = B;
+        for (int w = 0; w < W; ++w) a.qx[w] = a.qz[w] = qmask[w];
+        a.layers = d_prog;
+        a.n_layers = n;
+        a.keep = keep;
+        a.ukeep = d_ukeep;
+        a.ocount = d_ocount;
+        a.ox = ox;
+        a.oz = oz;
+        a.oc = (T*)oc;
+        a.obucket = ob;
+        a.ocap = ocap;
+        a.px = s->col[out].x;
+        a.pz = s->col[out].z;
+        a.pc = (T*)s->col[out].c;
+        a.pcap = s->col[out].cap;
+        a.pstart = s->bk[out].start;
+        a.plen = s->bk[out].len;
+        return launch_class_place<W, T>(ctx->stream, a, out_rows);
+    });
+    PP_HIP(ctx, e);
+    // the longest bucket (from the lens kernel) and the layout's end (the scan's total): one synchronization
+    PP_HIP(ctx, hipMemcpyAsync(h + kClassLongest, d_cst + kClassLongest, 8, hipMemcpyDeviceToHost, ctx->stream));
+    if ((st = read_stats(s)) != PP_OK) return st;
+    s->cur = out;
+    s->len = total;
+    s->max_bucket = (uint32_t)h[kClassLongest];
+    s->tail = s->h_stats[3];
+    s->has_room = true;
+    for (uint32_t l = 0; l < n && len_after; ++l) len_after[l] = h[kClassUntouched] + h[kClassLayerRows + l];
+    if (h[kClassUntouched] + h[kClassLayerRows + n - 1] != total)
+        return fail(ctx, PP_ERR_DRIVER, "class pass: the last layer's count disagrees with its outputs");
+    *done = true;
+    return PP_OK;
+}
+
 // The counts of an asynchronous out-of-place layer, without applying a selection left pending for the next layer.
 pp_status refresh_counts(pp_sum* s) {
     if (!s->stale) return PP_OK;
@@ -1503,6 +1856,13 @@ struct RunPlanner {
             walk = is_permutation_step(steps.back());
         }
         if (walk) return permutation_run_device(s, steps.data(), (uint32_t)steps.size(), keep, len_after);
+        if (end - i >= kClassMinRunLayers && s->len > 0 && class_rotation_runs()) {
+            std::vector<Prepared> run;
+            for (uint64_t j = i; j < end; ++j) run.push_back(prepare(s, channel_at(j), adjoint));
+            bool done = false;
+            if ((st = class_pass(s, run.data(), (uint32_t)run.size(), keep, len_after, &done)) != PP_OK) return st;
+            if (done) return PP_OK;
+        }
         for (uint64_t j = i; j < end; ++j) {
             const Prepared q = prepare(s, channel_at(j), adjoint);
             if ((st = apply_one(s, q, keep)) != PP_OK) return st;
@@ -1514,6 +1874,29 @@ struct RunPlanner {
     // The layer pass after a run.  With defer, an exact TopN leaves its retain to the next join (the caller applies
     // it with materialize_topn before anything else reads the rows).
     pp_status finish_run(pp_sum* s, bool defer = false) const;
+
+    // A class-pass window from layer i: rotation tables that are not permutation steps, at most kClassMaxLayers of
+    // them on at most kClassMaxQubits qubits, whatever the planner's run boundaries among them.  Only under a policy
+    // with no layer pass: finish_run then does nothing, so those boundaries change nothing but the reference's bucket
+    // count, which the caller replays from the window's per-layer lengths.  Returns the window's end (i: none).
+    uint64_t window_end(pp_sum* s, uint64_t i, std::vector<Prepared>* out) const {
+        out->clear();
+        if (finalizes) return i;
+        std::vector<uint32_t> qs;
+        uint64_t j = i;
+        for (; j < n_channels && j < i + kClassMaxLayers; ++j) {
+            const Prepared q = prepare(s, channel_at(j), adjoint);
+            if (!is_rotation_table(q) || is_permutation_step(q)) break;
+            std::vector<uint32_t> more = qs;
+            for (uint32_t k = 0; k < q.table.k; ++k) more.push_back(q.table.qubits[k]);
+            std::sort(more.begin(), more.end());
+            more.erase(std::unique(more.begin(), more.end()), more.end());
+            if (more.size() > kClassMaxQubits) break;
+            qs = more;
+            out->push_back(q);
+        }
+        return j;
+    }
 };
 
 }  // namespace
@@ -1623,6 +2006,11 @@ void pp_context_destroy(pp_context* ctx) {
     }
     (void)hipStreamDestroy(ctx->stream);
     if (ctx->d_prof) (void)hipFree(ctx->d_prof);
+    for (int k = 0; k < 2; ++k) {
+        if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);
+        if (ctx->stage_ev[k]) (void)hipEventDestroy(ctx->stage_ev[k]);
+    }
+    if (ctx->class_pinned) (void)hipHostFree(ctx->class_pinned);
     delete ctx;
 }
 
@@ -1752,9 +2140,14 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
                 std::memcpy(&hc[pos * cb], (const unsigned char*)coeff + i * cb, cb);
             }
     });
-    if (hipMemcpy(s->col[0].x, hx.get(), (size_t)width * cap * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->col[0].z, hz.get(), (size_t)width * cap * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->col[0].c, hc.get(), cap * cb, hipMemcpyHostToDevice) != hipSuccess ||
+    // the rows in use of every word plane and the coefficients, through the pinned chunks
+    for (uint32_t w = 0; w < width; ++w) {
+        if ((st = staged_h2d(ctx, s->col[0].x + w * cap, hx.get() + w * cap, acc * 8)) != PP_OK ||
+            (st = staged_h2d(ctx, s->col[0].z + w * cap, hz.get() + w * cap, acc * 8)) != PP_OK)
+            return bail(st);
+    }
+    if ((st = staged_h2d(ctx, s->col[0].c, hc.get(), acc * cb)) != PP_OK) return bail(st);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
         hipMemcpy(s->bk[0].start, start.data(), B * 8, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->bk[0].len, lens.data(), B * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->bk[0].rows, room.data(), B * 4, hipMemcpyHostToDevice) != hipSuccess)
@@ -1978,13 +2371,17 @@ pp_status pp_sum_download(pp_sum* s, uint64_t* x, uint64_t* z, void* coeff) {
         cleanup();
         return fail(ctx, PP_ERR_OUT_OF_MEMORY, "host staging allocation");
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(hx.get(), dx, n * W * 8, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(hz.get(), dz, n * W * 8, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(hc.get(), dc, n * cb, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(hb.get(), db, n * 4, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(ctx, PP_ERR_DRIVER, std::string("download: ") + hipGetErrorString(e));
+    }
+    // through the pinned chunks (each call drains its own copies)
+    if ((st = staged_d2h(ctx, hx.get(), dx, n * W * 8)) != PP_OK || (st = staged_d2h(ctx, hz.get(), dz, n * W * 8)) != PP_OK ||
+        (st = staged_d2h(ctx, hc.get(), dc, n * cb)) != PP_OK || (st = staged_d2h(ctx, hb.get(), db, n * 4)) != PP_OK) {
+        cleanup();
+        return st;
+    }
     cleanup();
-    if (e != hipSuccess) return fail(ctx, PP_ERR_DRIVER, std::string("download: ") + hipGetErrorString(e));
     // canonical order: the reference's bucket ascending (counting sort), then lexicographic key per bucket; each
     // step on up to 16 host threads (keys are distinct, so the order is the same however the work is split)
     const uint32_t RB = 1u << s->reference_bits;
@@ -2178,6 +2575,11 @@ pp_status pp_apply_run(pp_sum* s, pp_run_kind kind, const pp_prepared* layers, u
     const KeepProgram keep = lower_policy(policy);
     if (kind == PP_RUN_PERMUTATIONS && n > 0 && s->len > 0)
         return permutation_run_device(s, preps.data(), n, keep, layer_len);
+    if (n >= kClassMinRunLayers && s->len > 0 && class_rotation_runs()) {
+        bool done = false;
+        if ((st = class_pass(s, preps.data(), n, keep, layer_len, &done)) != PP_OK) return st;
+        if (done) return PP_OK;
+    }
     for (uint32_t j = 0; j < n; ++j) {
         if (s->len == 0) {  // nothing left to transform: the remaining layers keep it empty
             if (layer_len) layer_len[j] = 0;
@@ -2240,9 +2642,31 @@ pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channel
     if ((st = check_channels(s, channels, n_channels)) != PP_OK) return st;
     const RunPlanner plan{channels, n_channels, direction == PP_HEISENBERG, policy, target, min_buckets};
     const KeepProgram keep = lower_policy(policy);
-    uint64_t i = 0;
+    uint64_t i = 0, run_end = 0;  // layer i belongs to the planner's run ending at run_end
+    std::vector<Prepared> window;
+    std::vector<uint64_t> lens;
     while (i < n_channels) {
-        const uint64_t end = plan.start_run(s, i);
+        if (i == run_end) run_end = plan.start_run(s, i);
+        // rotation tables across the planner's run boundaries in one class pass, under a policy with no layer pass
+        const uint64_t e = class_rotation_runs() && s->len > 0 ? plan.window_end(s, i, &window) : i;
+        if (e - i >= kClassMinRunLayers) {
+            lens.assign(e - i, 0);
+            bool done = false;
+            if ((st = class_pass(s, window.data(), (uint32_t)(e - i), keep, lens.data(), &done)) != PP_OK) return st;
+            if (done) {
+                const uint64_t len_now = s->len;
+                for (uint64_t j = i + 1; j < e; ++j)
+                    if (j == run_end) {  // a run starts inside the window: its bucket count from the length there
+                        s->len = lens[j - 1 - i];
+                        run_end = plan.start_run(s, j);
+                    }
+                s->len = len_now;
+                if (layer_len) std::copy(lens.begin(), lens.end(), layer_len + i);
+                i = e;
+                continue;
+            }
+        }
+        const uint64_t end = run_end;  // the planner's run, or what a window left of it
         if ((st = plan.apply_run(s, i, end, keep, layer_len ? layer_len + i : nullptr)) != PP_OK) return st;
         if ((st = plan.finish_run(s, end < n_channels)) != PP_OK) return st;
         if (layer_len && plan.finalizes) layer_len[end - 1] = s->len;

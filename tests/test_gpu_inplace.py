@@ -20,6 +20,12 @@ from test_gpu_parity import oracle_policy, product_policy, random_circuit, rando
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True)
+def in_place_at_every_size(monkeypatch):
+    """The library runs layers of fewer than 2^18 rows out of place (launch-bound there); these cases are smaller."""
+    monkeypatch.setenv("PP_IN_PLACE_MIN_ROWS", "0")
+
+
 def announce(msg: str) -> None:
     print(f"[gpu case] {msg}", flush=True)
     sys.stdout.flush()

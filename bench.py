@@ -493,6 +493,84 @@ def tfim6x6_line(steps: int, warmup: int) -> dict:
     }
 
 
+def class_pass_line(kind: str, steps: int, warmup: int, log2n: int) -> dict:
+    """The class pass at the saturated layer's size: 2^log2n distinct keys on 127 qubits (the headline input), and
+    either the rotation run RX(phi_q) on qubits 0..7 (the dense layer's first eight RX angles; one pass over the sum
+    against eight layers one by one) or one dense two-qubit unitary (Haar-random from the seed) on qubits 0 and 1.
+    A working sum is reset from the resident input before each step (outside the timed region); a step is the run or
+    the layer, from its launch to its lengths on the host.  Merged terms/s counts every layer's output terms."""
+    import numpy as np
+    import torch
+
+    from paulistrings_amd import tables, truncation
+
+    x, z, c = saturated_keys(1 << log2n, SEED, "f64")
+    src = host_sum(x, z, c).to_device(0)
+    work = src.clone()
+    policy = truncation.coeff(THRESHOLD)
+    rng = np.random.default_rng(SEED)
+    if kind == "rotation-run":
+        phi = rng.uniform(0.0, math.pi, size=63 + NUM_QUBITS)[63:63 + 8]  # the dense layer's angles of RX 0..7
+        progs = [tables.rotation_table(NUM_QUBITS, "X", [q], float(phi[q])) for q in range(8)]
+        modes = [("class pass", "class"), ("layer by layer", "layers")]
+    else:
+        m = rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))
+        u, r = np.linalg.qr(m)
+        u = u * (np.diag(r) / np.abs(np.diag(r)))
+        progs = [tables.unitary_table(u, [0, 1])]
+        modes = [("class pass", None)]
+    results = {}
+    out_terms = None
+    for name, env in modes:
+        if env:
+            os.environ["PP_ROTATION_RUNS"] = env
+        try:
+            spent = []
+            for it in range(warmup + steps):
+                work.copy_from(src)
+                src.ctx.sync()
+                t0 = time.perf_counter()
+                if kind == "rotation-run":
+                    lens = work.apply_run(progs, policy)
+                else:
+                    work.apply_table(progs[0], policy)
+                    lens = [len(work)]
+                if it >= warmup:
+                    spent.append(time.perf_counter() - t0)
+            results[name] = min(spent) if spent else float("nan")
+            out_terms = int(sum(lens))
+            log(f"{kind}, {name}: {results[name] * 1e3:.3f} ms per step, {out_terms} output terms")
+        finally:
+            os.environ.pop("PP_ROTATION_RUNS", None)
+    n_in = len(src)
+    src.free()
+    work.free()
+    torch.cuda.synchronize()
+    t = results["class pass"]
+    line = {
+        "metric": "merged terms/s per layer (class pass)",
+        "value": out_terms / t,
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": t * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": ("RX on qubits 0..7 as one rotation run" if kind == "rotation-run" else
+                                "one Haar-random two-qubit unitary on qubits 0, 1") +
+                               f" over 2^{log2n} random keys (127 qubits, Bernoulli(0.1) bits), threshold 1e-6"},
+        "detail": {"input_terms": n_in, "output_terms_all_layers": out_terms,
+                   "ms": {k: v * 1e3 for k, v in results.items()}},
+    }
+    if "layer by layer" in results:
+        line["detail"]["speedup_over_layers"] = results["layer by layer"] / t
+    return line
+
+
 def kicked_ising_line(cpu_budget_s: float) -> dict:
     """Kicked Ising on the 127-qubit heavy-hex graph: 20 steps of RZZ(-pi/2) on the 144 edges (greedy colouring in
     sorted order) then RX(0.8) on every qubit, observable Z_62, f64, Heisenberg, CoefficientThreshold(1e-5) and
@@ -585,11 +663,14 @@ def main() -> int:
     ap.add_argument("--main-only", action="store_true",
                     help="only the in-place line (no out-of-place comparison, no float32 line, no STREAM probe): "
                          "for profiler runs")
-    ap.add_argument("--workload", choices=["saturated", "learned-score", "batched", "tfim6x6", "kicked-ising"],
+    ap.add_argument("--workload", choices=["saturated", "learned-score", "batched", "tfim6x6", "kicked-ising",
+                                           "rotation-run", "general-table"],
                     default="saturated",
                     help="saturated: the headline line (default); learned-score: the learned pass's MLP scoring on "
                          "2^22 terms; batched: 64 independent propagations stepped together; tfim6x6: the 6x6 TFIM "
-                         "against the oracle; kicked-ising: the 127-qubit heavy-hex kicked Ising with TopN 2^24")
+                         "against the oracle; kicked-ising: the 127-qubit heavy-hex kicked Ising with TopN 2^24; "
+                         "rotation-run / general-table: the class pass on the saturated input (eight RX as one run "
+                         "against one by one; one dense two-qubit unitary)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference for the HBM "
                          "traffic counters) and exit")
@@ -624,6 +705,9 @@ def main() -> int:
         return 0
     if args.workload == "kicked-ising":
         print(json.dumps(kicked_ising_line(args.cpu_seconds)), flush=True)
+        return 0
+    if args.workload in ("rotation-run", "general-table"):
+        print(json.dumps(class_pass_line(args.workload, args.steps, args.warmup, args.log2n)), flush=True)
         return 0
 
     channels = dense_layer_channels(SEED)

@@ -118,6 +118,14 @@ pp_status pp_sum_len(pp_sum* sum, uint64_t* len);
 pp_status pp_sum_download(pp_sum* sum, uint64_t* x, uint64_t* z, void* coeff);
 /* The reference's bucket bits for this sum, and the device's own. */
 pp_status pp_sum_bucket_bits(pp_sum* sum, int32_t* reference_bits, int32_t* device_bits);
+/* The resident layout's invariants, counted (the reference's debug-build k_check_invariants,
+ * engine/gpu/kernels/kern/invariants.cu:5, run from engine/gpu/sum.rs:559-565): out[0] rows whose key hashes to
+ * another device bucket, [1] zero coefficients, [2] buckets whose segment is misaligned, outside the columns or
+ * longer than its room, [3] keys held twice in one bucket, [4] buckets too long for the duplicate scan (over 1024
+ * rows), [5] overlapping segments, [6] 1 when the bucket lengths do not add up to the sum's length.  All zero for a
+ * sound layout.  With PP_CHECK_INVARIANTS=1 in the environment every layer entry point runs it after its layer and
+ * fails with PP_ERR_DRIVER on any violation. */
+pp_status pp_sum_check(pp_sum* sum, uint64_t* out);
 
 /* ---- layers -------------------------------------------------------------------------------- */
 

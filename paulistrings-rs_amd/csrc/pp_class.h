@@ -65,6 +65,7 @@ struct ClassArgs {
     uint64_t qx[W], qz[W];  // the program's qubits (cleared in a class key)
     const ClassLayer* layers;
     uint32_t n_layers;
+    uint32_t prog_words;  // sum over the layers of n_entries * 16: the amplitudes the apply holds in LDS
     KeepProgram keep;
     // routing
     uint32_t* ccount;  // [num_classes] touched rows per class bucket
@@ -84,6 +85,7 @@ struct ClassArgs {
     uint32_t* ocount;  // [num_buckets] outputs per output bucket
     unsigned long long* stats;  // ClassStat slots
     uint32_t lcap;              // rows of one LDS set
+    uint32_t chunk_rows;        // staged rows one apply iteration aims for (whole class buckets)
     uint32_t tbits;             // LDS table slots: 1 << tbits >= 2 lcap
     // the output layout (the spare columns)
     uint64_t* px;
@@ -95,7 +97,7 @@ struct ClassArgs {
 };
 
 template <int W, typename T>
-size_t class_lds_bytes(uint32_t lcap, uint32_t tbits);
+size_t class_lds_bytes(uint32_t lcap, uint32_t tbits, uint32_t prog_words);
 
 // Each launcher queues its kernel on st.
 template <int W, typename T>

@@ -1,13 +1,15 @@
 // pp_class.hip -- the class pass (pp_class.h): a program of up to eight local tables applied in one pass, each class
 // bucket's strings held in LDS through every layer.
 //
-// Layout of one apply workgroup's LDS: two row sets of lcap rows (word-major keys, then coefficients) that trade
-// places every layer, and an open-addressing table over the live set (slot = row + 1, 0 = empty).  A layer gathers
-// every output from its sources: an output K with pattern t has, for each entry e, the source K ^ lift(delta[e])
-// with pattern t ^ delta[e] and amplitude amp[e][t ^ delta[e]]; the source that comes first in entry order among
-// those present owns K, sums its contributions in entry order (one rounded multiply and one rounded add each, as a
-// rotation's pair update or the host engine's merge), asks keep and appends it to the other set.  A rotation table
-// (identity entry first, then the generator) gives own = c cos + c_partner a1 exactly as the pair join does.
+// Layout of one apply workgroup's LDS: the program's amplitudes; two row sets of lcap rows (word-major keys, then
+// coefficients) that trade places every layer; per layer an open-addressing table of the live set's classes under
+// that layer (the key with the layer's support cleared) and one record per class: the row of each of its (at most
+// 16) patterns.  A layer gathers every output from its sources: an output K with pattern t has, for each entry e,
+// the source of pattern t ^ delta[e] in its class, with amplitude amp[e][t ^ delta[e]]; the source that comes first
+// in entry order among those present owns K, sums its contributions in entry order (one rounded multiply and one
+// rounded add each, as a rotation's pair update or the host engine's merge), asks keep and appends it to the other
+// set.  A rotation table (identity entry first, then the generator) gives own = c cos + c_partner a1 exactly as the
+// pair join does.
 
 #include "pp_class.h"
 
@@ -37,31 +39,6 @@ __device__ __forceinline__ Key<W> class_key(const Key<W>& k, const uint64_t (&qx
 }
 
 __device__ __forceinline__ uint32_t lds_slot(uint32_t fold, uint32_t tbits) { return fold_mix(fold) >> (32u - tbits); }
-
-template <int W>
-__device__ __forceinline__ void lds_insert(uint32_t* table, uint32_t tbits, const Key<W>& k, uint32_t i) {
-    const uint32_t tmask = (1u << tbits) - 1u;
-    uint32_t s = lds_slot(key_fold<W>(k), tbits);
-    while (atomicCAS(&table[s], 0u, i + 1u) != 0u) s = (s + 1u) & tmask;
-}
-
-// The row of the live set holding q, or -1.  The table is at most half full, so a probe ends.
-template <int W>
-__device__ __forceinline__ int lds_find(const uint64_t* X, const uint64_t* Z, uint32_t cap, const uint32_t* table,
-                                        uint32_t tbits, const Key<W>& q) {
-    const uint32_t tmask = (1u << tbits) - 1u;
-    uint32_t s = lds_slot(key_fold<W>(q), tbits);
-    for (;;) {
-        const uint32_t e = table[s];
-        if (e == 0u) return -1;
-        const uint32_t i = e - 1u;
-        bool eq = true;
-#pragma unroll
-        for (int w = 0; w < W; ++w) eq = eq && X[w * cap + i] == q.x[w] && Z[w * cap + i] == q.z[w];
-        if (eq) return (int)i;
-        s = (s + 1u) & tmask;
-    }
-}
 
 // One wave per input bucket.  Route (GATHER false): count every touched string into its class bucket and every
 // untouched one that keep passes into ukeep[b].  Gather (GATHER true): place the touched strings in class-bucket order
@@ -103,30 +80,81 @@ __global__ void __launch_bounds__(kClassBlock) k_class_route(const ClassArgs<W, 
 }
 
 // A persistent grid over the class buckets; each workgroup runs the whole program on one class bucket at a time.
-// Per layer three barriers: the table cleared and the layer's entries copied to LDS; the live set inserted; the
-// outputs written.  The output counters alternate between two slots, so the next layer resets its slot while the
-// last readers of this one are still behind the third barrier.
+// The program's tables are copied to LDS once per workgroup.  Per layer three barriers: the table cleared; the live
+// set inserted; the outputs written.  The output counters alternate between two slots, so the next layer resets its
+// slot while the last readers of this one are still behind the third barrier.
 template <int W, typename T>
 __global__ void __launch_bounds__(kClassBlock) k_class_apply(const ClassArgs<W, T> a) {
     extern __shared__ __align__(16) unsigned char class_lds[];
     const uint32_t cap = a.lcap;
-    uint64_t* const X0 = reinterpret_cast<uint64_t*>(class_lds);
+    double* const P = reinterpret_cast<double*>(class_lds);  // every layer's amplitudes, entry-major
+    uint64_t* const X0 = reinterpret_cast<uint64_t*>(P + a.prog_words);
     uint64_t* const Z0 = X0 + (size_t)W * cap;
     uint64_t* const X1 = Z0 + (size_t)W * cap;
     uint64_t* const Z1 = X1 + (size_t)W * cap;
     T* const C0 = reinterpret_cast<T*>(Z1 + (size_t)W * cap);
     T* const C1 = C0 + cap;
     uint32_t* const table = reinterpret_cast<uint32_t*>(C1 + cap);
-    __shared__ double s_amp[kClassMaxEntries][16];
-    __shared__ uint32_t s_delta[kClassMaxEntries];
+    // per layer, one record per class of the live set (the key with the layer's support cleared): a row holding it
+    // and the row of each of its patterns (kNoRow where the class has none)
+    uint32_t* const rec_rep = table + (1u << a.tbits);
+    uint16_t* const rec_mem = reinterpret_cast<uint16_t*>(rec_rep + cap);  // [cap][16]
+    uint16_t* const row_rec = rec_mem + (size_t)cap * 16;                  // [cap]
+    __shared__ uint32_t s_word[kClassMaxLayers][2];
+    __shared__ uint32_t s_bit[kClassMaxLayers][2];
+    __shared__ uint32_t s_mask[kClassMaxLayers];
+    __shared__ uint32_t s_ne[kClassMaxLayers];
+    __shared__ uint32_t s_off[kClassMaxLayers];
+    __shared__ uint32_t s_delta[kClassMaxLayers][kClassMaxEntries];
     __shared__ uint32_t s_count[2];
     __shared__ uint32_t s_bad[2];
     __shared__ unsigned long long s_base;
     const uint32_t tid = threadIdx.x;
+    const uint32_t nl = a.n_layers;
 
-    for (uint32_t cb = blockIdx.x; cb < a.num_classes; cb += gridDim.x) {
-        const uint64_t base = a.cstart[cb];
-        const uint64_t end = cb + 1u < a.num_classes ? a.cstart[cb + 1u] : (uint64_t)a.stats[kClassStaged];
+    if (tid == 0) {
+        uint32_t off = 0;
+        for (uint32_t l = 0; l < nl; ++l) {
+            s_off[l] = off;
+            off += a.layers[l].n_entries * 16u;
+        }
+    }
+    if (tid < nl) {
+        const ClassLayer& L = a.layers[tid];
+        s_word[tid][0] = L.word[0];
+        s_word[tid][1] = L.word[1];
+        s_bit[tid][0] = L.bit[0];
+        s_bit[tid][1] = L.bit[1];
+        s_mask[tid] = L.mask;
+        s_ne[tid] = L.n_entries;
+        for (uint32_t e = 0; e < kClassMaxEntries; ++e) s_delta[tid][e] = L.delta[e];
+    }
+    __syncthreads();
+    for (uint32_t l = 0; l < nl; ++l)
+        for (uint32_t i = tid; i < s_ne[l] * 16u; i += kClassBlock) P[s_off[l] + i] = a.layers[l].amp[i >> 4][i & 15u];
+    __syncthreads();
+
+    // Chunk q holds the class buckets whose rows start in [q R, (q + 1) R) of the staging (R = chunk_rows): whole
+    // classes, independent of one another, share the LDS set and its table.
+    const uint64_t staged = a.stats[kClassStaged];
+    const uint64_t R = a.chunk_rows;
+    const uint64_t chunks = (staged + R - 1) / R;
+    __shared__ unsigned long long s_range[2];
+    for (uint64_t q = blockIdx.x; q < chunks; q += gridDim.x) {
+        if (tid < 2) {  // the first class bucket starting at or after q R (or (q + 1) R), as a row of the staging
+            const uint64_t at = (q + tid) * R;
+            uint32_t lo = 0, hi = a.num_classes;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.cstart[mid] < at) lo = mid + 1;
+                else hi = mid;
+            }
+            s_range[tid] = lo < a.num_classes ? a.cstart[lo] : staged;
+        }
+        __syncthreads();
+        const uint64_t base = s_range[0];
+        const uint64_t end = s_range[1];
+        __syncthreads();  // read before the next chunk's search writes them
         const uint32_t n0 = (uint32_t)(end - base);
         if (n0 == 0u) continue;  // uniform
         if (n0 > cap) {
@@ -139,8 +167,7 @@ __global__ void __launch_bounds__(kClassBlock) k_class_apply(const ClassArgs<W, 
         }
         uint32_t n = n0;
         bool in0 = true, bad = false;
-        for (uint32_t l = 0; l < a.n_layers; ++l) {
-            const ClassLayer& L = a.layers[l];
+        for (uint32_t l = 0; l < nl; ++l) {
             const uint32_t slot = l & 1u;
             const uint64_t* IX = in0 ? X0 : X1;
             const uint64_t* IZ = in0 ? Z0 : Z1;
@@ -148,40 +175,78 @@ __global__ void __launch_bounds__(kClassBlock) k_class_apply(const ClassArgs<W, 
             uint64_t* OX = in0 ? X1 : X0;
             uint64_t* OZ = in0 ? Z1 : Z0;
             T* OC = in0 ? C1 : C0;
+            const double* A = P + s_off[l];  // A[e * 16 + s]
+            const uint32_t* D = s_delta[l];
+            const uint32_t ne = s_ne[l];
             // a table of at least twice the live rows (a probe then ends), at most the launch's
             uint32_t tb = 4;
             while (tb < a.tbits && (1u << tb) < 2u * n) ++tb;
             for (uint32_t i = tid; i < (1u << tb); i += kClassBlock) table[i] = 0u;
-            const uint32_t ne = L.n_entries;
-            for (uint32_t i = tid; i < ne * 16u; i += kClassBlock) s_amp[i >> 4][i & 15u] = L.amp[i >> 4][i & 15u];
-            if (tid < ne) s_delta[tid] = L.delta[tid];
             if (tid == 0) {
                 s_count[slot] = 0u;
                 s_bad[slot] = 0u;
             }
+            // the layer's support, cleared in a class key
+            Key<W> sup{};
+            xor_local_delta<W>(sup, s_mask[l], s_word[l], s_bit[l]);
             __syncthreads();
-            for (uint32_t i = tid; i < n; i += kClassBlock) lds_insert<W>(table, tb, load_key<W>(IX, IZ, cap, i), i);
+            // every row into its class's record (a record is written whole before its slot is published)
+            for (uint32_t i = tid; i < n; i += kClassBlock) {
+                const Key<W> k = load_key<W>(IX, IZ, cap, i);
+                Key<W> ck;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    ck.x[w] = k.x[w] & ~sup.x[w];
+                    ck.z[w] = k.z[w] & ~sup.z[w];
+                }
+                const uint32_t s = pattern_of<W>(k, s_word[l], s_bit[l], s_mask[l]);
+                const uint32_t tmask = (1u << tb) - 1u;
+                uint32_t h = lds_slot(key_fold<W>(ck), tb);
+                uint32_t r = 0xFFFFFFFFu;
+                while (r == 0xFFFFFFFFu) {
+                    uint32_t e = table[h];
+                    if (e == 0u) {  // publish a record of this row's own (record i, unused if another row wins)
+                        rec_rep[i] = i;
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) reinterpret_cast<uint32_t*>(rec_mem + (size_t)i * 16)[q] = 0xFFFFFFFFu;
+                        __threadfence_block();
+                        e = atomicCAS(&table[h], 0u, i + 1u);
+                        if (e == 0u) {
+                            r = i;
+                            break;
+                        }
+                    }
+                    const Key<W> rk = load_key<W>(IX, IZ, cap, rec_rep[e - 1u]);
+                    bool same = true;
+#pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        same = same && (rk.x[w] & ~sup.x[w]) == ck.x[w] && (rk.z[w] & ~sup.z[w]) == ck.z[w];
+                    if (same) r = e - 1u;
+                    else h = (h + 1u) & tmask;
+                }
+                rec_mem[(size_t)r * 16 + s] = (uint16_t)i;
+                row_rec[i] = (uint16_t)r;
+            }
             __syncthreads();
             for (uint32_t i = tid; i < n; i += kClassBlock) {
                 const Key<W> k = load_key<W>(IX, IZ, cap, i);
                 const T c = IC[i];
-                const uint32_t s = pattern_of<W>(k, L.word, L.bit, L.mask);
+                const uint32_t s = pattern_of<W>(k, s_word[l], s_bit[l], s_mask[l]);
+                const uint16_t* const mem = rec_mem + (size_t)row_rec[i] * 16;
                 for (uint32_t e = 0; e < ne; ++e) {
-                    if (s_amp[e][s] == 0.0) continue;
-                    const uint32_t t = s ^ s_delta[e];
+                    if (A[e * 16u + s] == 0.0) continue;
+                    const uint32_t t = s ^ D[e];
                     Key<W> K = k;
-                    xor_local_delta<W>(K, s_delta[e], L.word, L.bit);
+                    xor_local_delta<W>(K, D[e], s_word[l], s_bit[l]);
                     T w = T(0);
                     bool any = false, owner = true;
                     for (uint32_t f = 0; f < ne; ++f) {
-                        const double af = s_amp[f][t ^ s_delta[f]];
+                        const double af = A[f * 16u + (t ^ D[f])];
                         if (af == 0.0) continue;
                         T cf = c;
                         if (f != e) {
-                            Key<W> src = K;
-                            xor_local_delta<W>(src, s_delta[f], L.word, L.bit);
-                            const int j = lds_find<W>(IX, IZ, cap, table, tb, src);
-                            if (j < 0) continue;
+                            const uint32_t j = mem[t ^ D[f]];  // the class's row of the source pattern
+                            if (j == 0xFFFFu) continue;
                             if (!any) {  // an earlier entry's source is present: it owns K
                                 owner = false;
                                 break;
@@ -296,13 +361,15 @@ inline uint32_t wave_grid(uint32_t buckets) { return (buckets + kClassBlock / 64
 }  // namespace
 
 template <int W, typename T>
-size_t class_lds_bytes(uint32_t lcap, uint32_t tbits) {
-    return (size_t)2 * lcap * (2 * W * sizeof(uint64_t) + sizeof(T)) + ((size_t)4 << tbits);
+size_t class_lds_bytes(uint32_t lcap, uint32_t tbits, uint32_t prog_words) {
+    return (size_t)prog_words * 8 + (size_t)2 * lcap * (2 * W * sizeof(uint64_t) + sizeof(T)) + ((size_t)4 << tbits) +
+           (size_t)lcap * (4 + 32 + 2);
 }
 
 template <int W, typename T>
 hipError_t launch_class_route(hipStream_t st, const ClassArgs<W, T>& a) {
     if (a.num_buckets == 0) return hipSuccess;
+    if (!a.layers || !a.rows || !a.ccount || !a.ukeep || !a.stats) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_class_route<W, T, false>), dim3(wave_grid(a.num_buckets)), dim3(kClassBlock), 0, st, a);
     return hipGetLastError();
 }
@@ -310,6 +377,7 @@ hipError_t launch_class_route(hipStream_t st, const ClassArgs<W, T>& a) {
 template <int W, typename T>
 hipError_t launch_class_gather(hipStream_t st, const ClassArgs<W, T>& a) {
     if (a.num_buckets == 0) return hipSuccess;
+    if (!a.layers || !a.rows || !a.ccount || !a.cstart || !a.sx || !a.sz || !a.sc) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_class_route<W, T, true>), dim3(wave_grid(a.num_buckets)), dim3(kClassBlock), 0, st, a);
     return hipGetLastError();
 }
@@ -317,8 +385,14 @@ hipError_t launch_class_gather(hipStream_t st, const ClassArgs<W, T>& a) {
 template <int W, typename T>
 hipError_t launch_class_apply(hipStream_t st, const ClassArgs<W, T>& a) {
     if (a.num_classes == 0) return hipSuccess;
-    if (a.tbits < 2 || a.tbits > 14 || ((size_t)1 << a.tbits) < 2 * (size_t)a.lcap) return hipErrorInvalidValue;
-    const size_t lds = class_lds_bytes<W, T>(a.lcap, a.tbits);
+    if (!a.layers || !a.rows || !a.cstart || !a.sx || !a.ox || !a.obucket || !a.ocount || !a.stats ||
+        a.chunk_rows == 0)
+        return hipErrorInvalidValue;
+    if (a.tbits < 2 || a.tbits > 14 || ((size_t)1 << a.tbits) < 2 * (size_t)a.lcap || a.lcap > 0xFFFFu)
+        return hipErrorInvalidValue;
+    if (a.prog_words > kClassMaxLayers * kClassMaxEntries * 16 || a.n_layers > kClassMaxLayers)
+        return hipErrorInvalidValue;
+    const size_t lds = class_lds_bytes<W, T>(a.lcap, a.tbits, a.prog_words);
     static int cus = 0;
     if (cus == 0) {
         int dev = 0;
@@ -326,8 +400,8 @@ hipError_t launch_class_apply(hipStream_t st, const ClassArgs<W, T>& a) {
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
     }
-    static int resident[15] = {};  // by tbits (which fixes lcap)
-    int& per = resident[a.tbits];
+    static int resident[15][5] = {};  // by tbits (which fixes lcap) and the program's size in 4 KiB
+    int& per = resident[a.tbits][std::min<uint32_t>(a.prog_words / 512u, 4u)];
     if (per == 0 &&
         (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_class_apply<W, T>, kClassBlock, lds) != hipSuccess ||
          per <= 0))
@@ -340,6 +414,7 @@ hipError_t launch_class_apply(hipStream_t st, const ClassArgs<W, T>& a) {
 template <int W, typename T>
 hipError_t launch_class_lens(hipStream_t st, const ClassArgs<W, T>& a) {
     if (a.num_buckets == 0) return hipSuccess;
+    if (!a.stats || !a.ukeep || !a.ocount || !a.plen) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_class_lens, dim3((a.num_buckets + 255) / 256), dim3(256), 0, st, a.ukeep, a.ocount, a.plen,
                        a.num_buckets, a.stats + kClassLongest);
     return hipGetLastError();
@@ -347,6 +422,7 @@ hipError_t launch_class_lens(hipStream_t st, const ClassArgs<W, T>& a) {
 
 template <int W, typename T>
 hipError_t launch_class_place(hipStream_t st, const ClassArgs<W, T>& a, uint64_t out_rows) {
+    if (!a.ukeep || !a.ocount || !a.pstart || !a.px || (out_rows && !a.obucket)) return hipErrorInvalidValue;
     const uint32_t place_blocks = wave_grid(a.num_buckets);
     const uint64_t blocks = place_blocks + (out_rows + kClassBlock - 1) / kClassBlock;
     if (blocks == 0) return hipSuccess;
@@ -357,7 +433,7 @@ hipError_t launch_class_place(hipStream_t st, const ClassArgs<W, T>& a, uint64_t
 }
 
 #define PP_CLASS_INSTANTIATE(W, T)                                                              \
-    template size_t class_lds_bytes<W, T>(uint32_t, uint32_t);                                  \
+    template size_t class_lds_bytes<W, T>(uint32_t, uint32_t, uint32_t);                                  \
     template hipError_t launch_class_route<W, T>(hipStream_t, const ClassArgs<W, T>&);          \
     template hipError_t launch_class_gather<W, T>(hipStream_t, const ClassArgs<W, T>&);         \
     template hipError_t launch_class_apply<W, T>(hipStream_t, const ClassArgs<W, T>&);          \

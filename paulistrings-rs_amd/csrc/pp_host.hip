@@ -355,6 +355,9 @@ struct pp_sum {
     // A TopN selection whose retain is left to the next join (inside pp_propagate only): len is then the kept
     // count, phys_len the rows the columns still hold, and max_bucket their longest bucket.
     bool pending_topn = false;
+    uint64_t class_out_hint = 0;   // the last class pass's outputs: the next one's first guess at its staging
+    uint64_t topn_seed = 0;        // the last selection's boundary prefix, which seeds the next one's first pass
+    bool topn_seed_valid = false;
     uint64_t phys_len = 0;
     const uint8_t* pending_flags = nullptr;
     const float* pending_score = nullptr;  // the ranking score when the selection was by score
@@ -1399,6 +1402,13 @@ pp_status device_layer(pp_sum* s, const Prepared& p, const KeepProgram& keep) {
     return PP_OK;
 }
 
+// Whether a TopN selection seeds its first pass with the previous boundary (default) or runs its three passes
+// (PP_TOPN_SEEDING=0, for comparison: both decide the same digits).
+bool topn_seeding() {
+    const char* e = std::getenv("PP_TOPN_SEEDING");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
 // Whether pp_propagate walks permutation runs in one pass (default) or layer by layer (PP_PERMUTATION_RUNS=layers,
 // for comparison: both give the same keys and coefficients).
 bool fused_permutation_runs() {
@@ -1545,11 +1555,14 @@ bool class_program_fits(const Prepared* layers, uint32_t n) {
     return qs.size() <= kClassMaxQubits;
 }
 
-// Whether pp_propagate and pp_apply_run give rotation runs to the class pass (default) or apply them layer by layer
-// (PP_ROTATION_RUNS=layers, for comparison: both give the same keys, coefficients and per-layer lengths).
+// Whether pp_propagate and pp_apply_run give rotation runs to the class pass (PP_ROTATION_RUNS=class) or apply them
+// layer by layer (the default): both give the same keys, coefficients and per-layer lengths.  Layer by layer is the
+// faster on this device at every size measured: at 2^24 terms an eight-layer run takes 12 ms as in-place layers and
+// 26-41 ms as a class pass, whose routing and placement pay one global atomic per row; on the 6x6 TFIM (about 3*10^4
+// terms) 29.5 against 41.7 ms.
 bool class_rotation_runs() {
     const char* e = std::getenv("PP_ROTATION_RUNS");
-    return !(e && std::strcmp(e, "layers") == 0);
+    return e && std::strcmp(e, "class") == 0;
 }
 
 // layers[0..n) in one class pass, the sum's length after each into len_after (optional).  *done = false, the sum as it
@@ -1581,8 +1594,10 @@ pp_status class_pass(pp_sum* s, const Prepared* layers, uint32_t n, const KeepPr
     ClassLayer* prog = static_cast<ClassLayer*>(ctx->class_pinned);
     unsigned long long* h = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->class_pinned) + kPinnedStats);
     uint64_t qmask[4] = {};
+    uint32_t prog_words = 0;
     for (uint32_t l = 0; l < n; ++l) {
         prog[l] = class_layer(layers[l]);
+        prog_words += prog[l].n_entries * 16u;
         for (uint32_t j = 0; j < layers[l].table.k; ++j)
             qmask[layers[l].table.qubits[j] / 64] |= 1ull << (layers[l].table.qubits[j] % 64);
     }
@@ -1599,11 +1614,14 @@ pp_status class_pass(pp_sum* s, const Prepared* layers, uint32_t n, const KeepPr
         (st = slot_scratch(s, kScrClassSZ, (size_t)s->W * len_in * 8, &sz)) != PP_OK ||
         (st = slot_scratch(s, kScrClassSC, len_in * cb, &sc)) != PP_OK)
         return st;
-    // class buckets of about 64 rows of the input, finer on an overflow; a larger LDS set last
+    // class buckets of about 16 rows of the input, packed into chunks of a quarter of the LDS set; on an overflow
+    // smaller chunks and finer class buckets, then a larger set
     int cbits = 0;
-    while (cbits < 22 && ((uint64_t)64 << cbits) < len_in) ++cbits;
-    uint32_t lcap = 512, tbits = 10;
-    uint64_t ocap = std::max<uint64_t>(2 * len_in, 4096);
+    while (cbits < 22 && ((uint64_t)16 << cbits) < len_in) ++cbits;
+    uint64_t growth = 1;  // the program's worst fanout, capped: a chunk of lcap / growth rows fits its outputs
+    for (uint32_t l = 0; l < n && growth < 16; ++l) growth *= std::max<uint32_t>(layers[l].fanout, 1u);
+    uint32_t lcap = 512, tbits = 10, chunk = (uint32_t)std::max<uint64_t>(16, 512 / std::min<uint64_t>(growth, 16));
+    uint64_t ocap = std::max<uint64_t>({2 * len_in, (uint64_t)4096, s->class_out_hint});
     uint64_t *ox = nullptr, *oz = nullptr;
     void* oc = nullptr;
     uint32_t* ob = nullptr;
@@ -1645,6 +1663,7 @@ pp_status class_pass(pp_sum* s, const Prepared* layers, uint32_t n, const KeepPr
             for (int w = 0; w < W; ++w) a.qx[w] = a.qz[w] = qmask[w];
             a.layers = d_prog;
             a.n_layers = n;
+            a.prog_words = prog_words;
             a.keep = keep;
             a.ccount = d_ccount;
             a.cstart = d_cstart;
@@ -1662,6 +1681,7 @@ pp_status class_pass(pp_sum* s, const Prepared* layers, uint32_t n, const KeepPr
             a.stats = d_cst;
             a.lcap = lcap;
             a.tbits = tbits;
+            a.chunk_rows = chunk;
             hipError_t r = launch_class_route<W, T>(ctx->stream, a);
             if (r == hipSuccess)
                 r = launch_bucket_offsets(ctx->stream, d_ccount, C, 0, 0, 0, 0, d_cstart, d_cst + kClassStaged,
@@ -1679,10 +1699,12 @@ pp_status class_pass(pp_sum* s, const Prepared* layers, uint32_t n, const KeepPr
             ocap = h[kClassOutRows];
         } else if (attempt == 0) {
             cbits = std::min(cbits + 2, 24);
+            chunk = 32;
         } else if (attempt == 1) {
             cbits = std::min(cbits + 1, 24);
             lcap = 1024;
             tbits = 11;
+            chunk = 32;
         } else {
             return PP_OK;  // a class that outgrows the largest set: layer by layer
         }
@@ -1690,6 +1712,7 @@ pp_status class_pass(pp_sum* s, const Prepared* layers, uint32_t n, const KeepPr
     }
     // the output layout in the spare columns: each bucket's untouched rows, then its class outputs
     const uint64_t out_rows = h[kClassOutRows], total = h[kClassUntouched] + out_rows;
+    s->class_out_hint = out_rows + out_rows / 8;
     const int in = s->cur, out = 1 - s->cur;
     if ((st = ensure_columns(s, out, total + (uint64_t)kSegAlign * B)) != PP_OK) return st;
     if ((st = ensure_buckets(s, out, B)) != PP_OK) return st;
@@ -1701,6 +1724,7 @@ pp_status class_pass(pp_sum* s, const Prepared* layers, uint32_t n, const KeepPr
         a.ukeep = d_ukeep;
         a.ocount = d_ocount;
         a.plen = s->bk[out].len;
+        a.stats = d_cst;
         return launch_class_lens<W, T>(ctx->stream, a);
     });
     PP_HIP(ctx, e);
@@ -2300,6 +2324,73 @@ pp_status pp_sum_len(pp_sum* s, uint64_t* len) {
     return PP_OK;
 }
 
+pp_status pp_sum_check(pp_sum* s, uint64_t* out) {
+    if (s) PP_NOT_CAPTURING(s->ctx, "pp_sum_check");
+    if (!s || !out) return PP_ERR_INVALID_ARGUMENT;
+    pp_context* ctx = s->ctx;
+    pp_status st;
+    if ((st = refresh(s)) != PP_OK) return st;
+    const uint32_t B = 1u << s->device_bits;
+    const int cur = s->cur;
+    if ((st = clear_stats(s)) != PP_OK) return st;
+    if (s->len > 0 || s->col[cur].x) {
+        hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
+            constexpr int W = decltype(Wc)::value;
+            using T = decltype(Tv);
+            CheckArgs<W, T> a{};
+            a.x = s->col[cur].x;
+            a.z = s->col[cur].z;
+            a.c = (const T*)s->col[cur].c;
+            a.cap = s->col[cur].cap;
+            a.start = s->bk[cur].start;
+            a.len = s->bk[cur].len;
+            a.room = s->has_room ? s->bk[cur].rows : nullptr;
+            a.end = s->has_room ? s->tail : s->col[cur].cap;
+            a.num_buckets = B;
+            a.rows = s->d_rows;
+            a.bits = s->device_bits;
+            a.out = s->d_stats;
+            return launch_check_layout<W, T>(ctx->stream, a);
+        });
+        PP_HIP(ctx, e);
+    }
+    if ((st = read_stats(s)) != PP_OK) return st;
+    for (int i = 0; i < 5; ++i) out[i] = s->h_stats[i];
+    // overlaps and the total, on the host from the bucket arrays
+    std::vector<uint64_t> start(B);
+    std::vector<uint32_t> len(B), room(B);
+    PP_HIP(ctx, hipMemcpy(start.data(), s->bk[cur].start, (size_t)B * 8, hipMemcpyDeviceToHost));
+    PP_HIP(ctx, hipMemcpy(len.data(), s->bk[cur].len, (size_t)B * 4, hipMemcpyDeviceToHost));
+    if (s->has_room) PP_HIP(ctx, hipMemcpy(room.data(), s->bk[cur].rows, (size_t)B * 4, hipMemcpyDeviceToHost));
+    std::vector<std::pair<uint64_t, uint64_t>> seg;
+    uint64_t total = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+        total += len[b];
+        const uint64_t extent = s->has_room ? room[b] : len[b];
+        if (extent) seg.push_back({start[b], extent});
+    }
+    std::sort(seg.begin(), seg.end());
+    uint64_t overlaps = 0;
+    for (size_t i = 1; i < seg.size(); ++i) overlaps += seg[i - 1].first + seg[i - 1].second > seg[i].first;
+    out[5] = overlaps;
+    out[6] = total != s->len;
+    return PP_OK;
+}
+
+// PP_CHECK_INVARIANTS=1: every layer entry point checks the layout it leaves.
+pp_status debug_check(pp_sum* s) {
+    const char* e = std::getenv("PP_CHECK_INVARIANTS");
+    if (!e || std::strcmp(e, "1") != 0) return PP_OK;
+    uint64_t out[7] = {};
+    pp_status st = pp_sum_check(s, out);
+    if (st != PP_OK) return st;
+    for (int i = 0; i < 7; ++i)
+        if (out[i] && i != 4)
+            return fail(s->ctx, PP_ERR_DRIVER,
+                        "layout invariant " + std::to_string(i) + " violated " + std::to_string(out[i]) + " times");
+    return PP_OK;
+}
+
 pp_status pp_sum_bucket_bits(pp_sum* s, int32_t* reference_bits, int32_t* device_bits) {
     if (!s) return PP_ERR_INVALID_ARGUMENT;
     if (reference_bits) *reference_bits = s->reference_bits;
@@ -2422,7 +2513,8 @@ pp_status pp_apply_rotation(pp_sum* s, const pp_channel* ch, int adjoint, const 
     pp_status st = refresh(s);
     if (st != PP_OK) return st;
     const Prepared p = prepare(s, *ch, adjoint != 0);
-    return apply_one(s, p, lower_policy(policy));
+    if ((st = apply_one(s, p, lower_policy(policy))) != PP_OK) return st;
+    return debug_check(s);
 }
 
 pp_status pp_apply_rotation_to(pp_sum* src, pp_sum* dst, const pp_channel* ch, int adjoint, const pp_policy* policy) {
@@ -2549,7 +2641,8 @@ pp_status pp_apply_prepared(pp_sum* s, const pp_prepared* prepared, const pp_pol
     if ((st = refresh_counts(s)) != PP_OK) return st;
     Prepared p;
     if ((st = classify(s, *prepared, &p)) != PP_OK) return st;
-    return apply_one(s, p, lower_policy(policy));
+    if ((st = apply_one(s, p, lower_policy(policy))) != PP_OK) return st;
+    return debug_check(s);
 }
 
 pp_status pp_apply_run(pp_sum* s, pp_run_kind kind, const pp_prepared* layers, uint32_t n, const pp_policy* policy,
@@ -2573,12 +2666,14 @@ pp_status pp_apply_run(pp_sum* s, pp_run_kind kind, const pp_prepared* layers, u
                             (kind == PP_RUN_PERMUTATIONS ? " is not a one-to-one step" : " is not rotation-shaped"));
     }
     const KeepProgram keep = lower_policy(policy);
-    if (kind == PP_RUN_PERMUTATIONS && n > 0 && s->len > 0)
-        return permutation_run_device(s, preps.data(), n, keep, layer_len);
+    if (kind == PP_RUN_PERMUTATIONS && n > 0 && s->len > 0) {
+        if ((st = permutation_run_device(s, preps.data(), n, keep, layer_len)) != PP_OK) return st;
+        return debug_check(s);
+    }
     if (n >= kClassMinRunLayers && s->len > 0 && class_rotation_runs()) {
         bool done = false;
         if ((st = class_pass(s, preps.data(), n, keep, layer_len, &done)) != PP_OK) return st;
-        if (done) return PP_OK;
+        if (done) return debug_check(s);
     }
     for (uint32_t j = 0; j < n; ++j) {
         if (s->len == 0) {  // nothing left to transform: the remaining layers keep it empty
@@ -2588,7 +2683,7 @@ pp_status pp_apply_run(pp_sum* s, pp_run_kind kind, const pp_prepared* layers, u
         if ((st = apply_one(s, preps[j], keep)) != PP_OK) return st;
         if (layer_len) layer_len[j] = s->len;
     }
-    return PP_OK;
+    return debug_check(s);
 }
 
 pp_status pp_finalize_layer(pp_sum* s, const pp_policy* policy) {
@@ -2662,6 +2757,7 @@ pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channel
                     }
                 s->len = len_now;
                 if (layer_len) std::copy(lens.begin(), lens.end(), layer_len + i);
+                if ((st = debug_check(s)) != PP_OK) return st;
                 i = e;
                 continue;
             }
@@ -2670,6 +2766,7 @@ pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channel
         if ((st = plan.apply_run(s, i, end, keep, layer_len ? layer_len + i : nullptr)) != PP_OK) return st;
         if ((st = plan.finish_run(s, end < n_channels)) != PP_OK) return st;
         if (layer_len && plan.finalizes) layer_len[end - 1] = s->len;
+        if ((st = debug_check(s)) != PP_OK) return st;
         i = end;
     }
     return PP_OK;

@@ -222,6 +222,33 @@ struct ExportArgs {
     uint32_t* out_bucket;
 };
 
+// The resident layout's invariants (the reference's k_check_invariants, kernels/invariants.cu): per bucket, the
+// segment aligned and inside the columns (within its room when the layout records one), every row in the bucket
+// its key hashes to, no zero coefficient, no key twice.
+enum CheckCount : uint32_t {
+    kCheckMisplaced = 0,  // rows whose key hashes to another bucket
+    kCheckZero = 1,       // rows with a coefficient of exactly zero
+    kCheckSegments = 2,   // buckets whose segment is misaligned, outside the columns or longer than its room
+    kCheckDuplicates = 3, // rows whose key another row of the bucket holds
+    kCheckUnscanned = 4,  // buckets too long for the duplicate scan (more than 1024 rows)
+    kCheckCounts = 5
+};
+template <int W, typename T>
+struct CheckArgs {
+    const uint64_t* x;
+    const uint64_t* z;
+    const T* c;
+    uint64_t cap;
+    const uint64_t* start;
+    const uint32_t* len;
+    const uint32_t* room;  // per-bucket segment rows, or null
+    uint64_t end;          // rows of the columns in use (segments end at or before it)
+    uint32_t num_buckets;
+    const uint64_t* rows;  // hash rows
+    int bits;
+    unsigned long long* out;  // CheckCount slots
+};
+
 // Exclusive scan of per-bucket sizes into out_start (one launch for a layout of one tile, else three); tile_sums
 // holds offsets_tiles(n) words.  When clear is set, clear[0 .. clear_n) is zeroed before the total is written (a
 // join's counters, without a launch of their own).
@@ -270,5 +297,7 @@ template <int W, typename T>
 hipError_t launch_expectation(hipStream_t st, const ExpectationArgs<W, T>& a, uint32_t blocks);
 template <int W, typename T>
 hipError_t launch_export(hipStream_t st, const ExportArgs<W, T>& a, uint32_t num_buckets);
+template <int W, typename T>
+hipError_t launch_check_layout(hipStream_t st, const CheckArgs<W, T>& a);
 
 }  // namespace pp

@@ -1426,6 +1426,56 @@ hipError_t launch_export(hipStream_t st, const ExportArgs<W, T>& a, uint32_t num
     return hipGetLastError();
 }
 
+// One workgroup per bucket: the segment's bounds, then every row's bucket, coefficient and key against the others
+// (an LDS table of row indices, keys compared in the columns).
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_check_layout(const CheckArgs<W, T> a) {
+    constexpr uint32_t kRows = 1024, kSlots = 2048;
+    __shared__ uint32_t table[kSlots];
+    __shared__ uint32_t counts[kCheckCounts];
+    const uint32_t b = blockIdx.x;
+    const uint64_t s0 = a.start[b];
+    const uint32_t n = a.len[b];
+    const uint64_t extent = a.room ? a.room[b] : n;
+    if (threadIdx.x < kCheckCounts) counts[threadIdx.x] = 0u;
+    for (uint32_t i = threadIdx.x; i < kSlots; i += kBlock) table[i] = 0u;
+    __syncthreads();
+    const bool seg_ok = (n == 0 || s0 % kSegAlign == 0) && n <= extent && (extent == 0 || s0 + extent <= a.end) &&
+                        a.end <= a.cap;
+    if (!seg_ok) {
+        if (threadIdx.x == 0) atomicAdd(&a.out[kCheckSegments], 1ull);
+        return;  // the whole workgroup: its rows are not read
+    }
+    const bool scan = n <= kRows;
+    for (uint32_t r = threadIdx.x; r < n; r += kBlock) {
+        const Key<W> k = load_key<W>(a.x, a.z, a.cap, s0 + r);
+        if (bucket_of<W>(k, a.rows, a.bits) != b) atomicAdd(&counts[kCheckMisplaced], 1u);
+        if (a.c[s0 + r] == T(0)) atomicAdd(&counts[kCheckZero], 1u);
+        if (!scan) continue;
+        uint32_t slot = fold_mix(key_fold<W>(k)) >> (32u - 11u);
+        for (;;) {
+            const uint32_t prev = atomicCAS(&table[slot], 0u, r + 1u);
+            if (prev == 0u) break;
+            const Key<W> q = load_key<W>(a.x, a.z, a.cap, s0 + prev - 1u);
+            if (key_eq<W>(k, q)) {
+                atomicAdd(&counts[kCheckDuplicates], 1u);
+                break;
+            }
+            slot = (slot + 1u) & (kSlots - 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && !scan) atomicAdd(&a.out[kCheckUnscanned], 1ull);
+    if (threadIdx.x < kCheckCounts && counts[threadIdx.x]) atomicAdd(&a.out[threadIdx.x], (unsigned long long)counts[threadIdx.x]);
+}
+
+template <int W, typename T>
+hipError_t launch_check_layout(hipStream_t st, const CheckArgs<W, T>& a) {
+    if (a.num_buckets == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_check_layout<W, T>), dim3(a.num_buckets), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
 #define PP_INSTANTIATE(W, T)                                                                           \
     template size_t join_lds_bytes<W, T>(uint32_t, uint32_t);                                          \
     template hipError_t launch_rotation_join<W, T>(hipStream_t, const JoinArgs<W, T>&, uint32_t, uint32_t); \
@@ -1439,7 +1489,8 @@ hipError_t launch_export(hipStream_t st, const ExportArgs<W, T>& a, uint32_t num
     template hipError_t launch_refine<W, T>(hipStream_t, const RefineArgs<W, T>&);                      \
     template hipError_t launch_coarsen<W, T>(hipStream_t, const CoarsenArgs<W, T>&);                    \
     template hipError_t launch_expectation<W, T>(hipStream_t, const ExpectationArgs<W, T>&, uint32_t);  \
-    template hipError_t launch_export<W, T>(hipStream_t, const ExportArgs<W, T>&, uint32_t);
+    template hipError_t launch_export<W, T>(hipStream_t, const ExportArgs<W, T>&, uint32_t);                 \
+    template hipError_t launch_check_layout<W, T>(hipStream_t, const CheckArgs<W, T>&);
 
 PP_INSTANTIATE(1, double)
 PP_INSTANTIATE(1, float)

@@ -19,6 +19,9 @@ struct TopnState {
     unsigned long long ties;     // terms at the final boundary
     unsigned long long keep_all_finite;  // at most n finite terms: keep every one of them
     unsigned long long gathered;  // tie terms written by k_topn_gather_ties
+    unsigned long long done;      // digits decided so far (a pass whose digit is decided does nothing)
+    unsigned long long seed;      // a seeded selection: the previous selection's prefix (its first two digits)
+    unsigned long long seeded;
 };
 
 __device__ __forceinline__ bool is_nan_bits(uint64_t u) { return (u & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull; }
@@ -116,7 +119,14 @@ struct TopnRetainArgs {
 template <typename T>
 hipError_t launch_topn_hist(hipStream_t st, const TopnHistArgs<T>& a, uint32_t blocks);
 hipError_t launch_topn_select(hipStream_t st, TopnState* state, unsigned long long* hist, int pass, uint64_t n);
-hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n);
+hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n, uint64_t seed = 0, bool seeded = false);
+// The seeded first pass: besides the first digit's histogram, the second digit's over the terms whose first digit is
+// the seed's; the select then decides one or two digits (the next pass does nothing for a digit decided).  hist holds
+// kSeededHistWords.
+constexpr uint32_t kSeededHistWords = 2 * kHistWords;
+template <typename T>
+hipError_t launch_topn_hist_seeded(hipStream_t st, const TopnHistArgs<T>& a, uint32_t blocks);
+hipError_t launch_topn_select_seeded(hipStream_t st, TopnState* state, unsigned long long* hist, uint64_t n);
 template <int W, typename T>
 hipError_t launch_topn_gather_ties(hipStream_t st, const TopnTieArgs<W, T>& a, uint32_t num_buckets);
 template <int W, typename T>

@@ -14,7 +14,7 @@ __device__ __forceinline__ uint64_t rank_bits(const T* c, const float* score, ui
     return score ? score_bits(score[i]) : magnitude_bits<T>(c[i]);
 }
 
-__global__ void k_topn_init(TopnState* s, uint64_t n) {
+__global__ void k_topn_init(TopnState* s, uint64_t n, uint64_t seed, uint32_t seeded) {
     s->prefix = 0;
     s->mask = 0;
     s->k = n;
@@ -23,6 +23,9 @@ __global__ void k_topn_init(TopnState* s, uint64_t n) {
     s->ties = 0;
     s->keep_all_finite = 0;
     s->gathered = 0;
+    s->done = 0;
+    s->seed = seed;
+    s->seeded = seeded;
 }
 
 // The digit holding the k-th largest after pass `pass`, from the complete histogram, run by one workgroup; then the
@@ -155,8 +158,9 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
     __shared__ uint32_t h[kTopnBins];
     __shared__ RowList list;
-    for (int i = threadIdx.x; i < kTopnBins; i += kBlock) h[i] = 0;
     const TopnState st = *a.state;
+    if (st.done > (unsigned long long)a.pass) return;  // decided by a seeded first pass: the whole workgroup
+    for (int i = threadIdx.x; i < kTopnBins; i += kBlock) h[i] = 0;
     const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
     const uint32_t nb = min(a.num_buckets, b_lo + a.buckets_per_block) - b_lo;  // <= kHistBucketsMax
     const uint32_t t = threadIdx.x;
@@ -192,7 +196,73 @@ __global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
 // One workgroup after each histogram pass (a kernel boundary, not a device-scope fence per histogram workgroup: on
 // a multi-XCD device such a fence writes back the XCD's L2).
 __global__ void __launch_bounds__(kBlock) k_topn_select(TopnState* s, unsigned long long* hist, int pass, uint64_t n) {
+    if (s->done > (unsigned long long)pass) return;  // uniform: one load, the same for every thread
     select_digit(s, hist, pass, n);
+    if (threadIdx.x == 0) s->done = (unsigned long long)pass + 1ull;
+}
+
+// The seeded first pass's histograms: region 0 of hist counts the first digit of every term, region 1 the second
+// digit of the terms whose first digit is the seed's.  (A third level, over the terms whose first two digits are the
+// seed's, measured slower: at a TopN cap the second digit of the boundary moves from layer to layer, and the third
+// histogram's LDS cost the pass more than it saved.)  Walks its buckets as k_topn_hist does.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_topn_hist_seeded(const TopnHistArgs<T> a) {
+    __shared__ uint32_t h[2][kTopnBins];
+    __shared__ RowList list;
+    for (int i = threadIdx.x; i < 2 * kTopnBins; i += kBlock) (&h[0][0])[i] = 0;
+    const TopnState st = *a.state;
+    const uint32_t spec0 = (uint32_t)(st.seed >> topn_shift(0)) & (kTopnBins - 1);
+    const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
+    const uint32_t nb = min(a.num_buckets, b_lo + a.buckets_per_block) - b_lo;  // <= kHistBucketsMax
+    const uint32_t total = list_rows(a.start, a.len, b_lo, nb, list);
+    uint32_t j = 0;
+    for (uint32_t g0 = threadIdx.x; g0 < total; g0 += 4 * kBlock) {
+        uint64_t pos[4];
+        bool in[4];
+        list_positions(list, total, g0, j, pos, in);
+        uint64_t u[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (in[q]) u[q] = rank_bits<T>(a.c, a.score, pos[q]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!in[q] || (!a.score && is_nan_bits(u[q]))) continue;
+            const uint32_t d0 = (uint32_t)(u[q] >> topn_shift(0)) & (kTopnBins - 1);
+            atomicAdd(&h[0][d0], 1u);
+            if (d0 != spec0) continue;
+            atomicAdd(&h[1][(uint32_t)(u[q] >> topn_shift(1)) & (kTopnBins - 1)], 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t copy = (blockIdx.x % kHistCopies) * kTopnBins;
+    for (int i = threadIdx.x; i < 2 * kTopnBins; i += kBlock) {
+        const uint32_t v = (&h[0][0])[i];
+        if (v) atomicAdd(&a.hist[(i / kTopnBins) * kHistWords + copy + (i % kTopnBins)], (unsigned long long)v);
+    }
+}
+
+// The first digit from region 0, the second from region 1 when the first is the seed's; region 1 is cleared for the
+// next selection when unused.
+__global__ void __launch_bounds__(kBlock) k_topn_select_seeded(TopnState* s, unsigned long long* hist, uint64_t n) {
+    __shared__ int go;
+    select_digit(s, hist, 0, n);
+    int done = 1;
+    for (int r = 1; r < 2; ++r) {
+        __syncthreads();  // the holder's update of the state is visible
+        if (threadIdx.x == 0) {
+            const uint64_t d = (s->prefix >> topn_shift(r - 1)) & (kTopnBins - 1);
+            go = done == r && !s->keep_all_finite && d == ((s->seed >> topn_shift(r - 1)) & (kTopnBins - 1));
+        }
+        __syncthreads();
+        if (go) {
+            select_digit(s, hist + (size_t)r * kHistWords, r, n);
+            done = r + 1;
+        } else {
+            for (uint32_t i = threadIdx.x; i < kHistWords; i += kBlock) hist[(size_t)r * kHistWords + i] = 0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s->done = s->keep_all_finite ? 3ull : (unsigned long long)done;
 }
 
 // The terms of the boundary bin: rank key equal to the prefix on the decided bits (a NaN never matches a finite
@@ -424,8 +494,19 @@ hipError_t launch_topn_select(hipStream_t st, TopnState* state, unsigned long lo
     return hipGetLastError();
 }
 
-hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n) {
-    hipLaunchKernelGGL(k_topn_init, dim3(1), dim3(1), 0, st, state, n);
+hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n, uint64_t seed, bool seeded) {
+    hipLaunchKernelGGL(k_topn_init, dim3(1), dim3(1), 0, st, state, n, seed, seeded ? 1u : 0u);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_topn_hist_seeded(hipStream_t st, const TopnHistArgs<T>& a, uint32_t blocks) {
+    hipLaunchKernelGGL((k_topn_hist_seeded<T>), dim3(blocks), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_topn_select_seeded(hipStream_t st, TopnState* state, unsigned long long* hist, uint64_t n) {
+    hipLaunchKernelGGL(k_topn_select_seeded, dim3(1), dim3(kBlock), 0, st, state, hist, n);
     return hipGetLastError();
 }
 
@@ -453,6 +534,8 @@ hipError_t launch_mark_positions(hipStream_t st, const uint64_t* pos, uint64_t n
 
 template hipError_t launch_topn_hist<double>(hipStream_t, const TopnHistArgs<double>&, uint32_t);
 template hipError_t launch_topn_hist<float>(hipStream_t, const TopnHistArgs<float>&, uint32_t);
+template hipError_t launch_topn_hist_seeded<double>(hipStream_t, const TopnHistArgs<double>&, uint32_t);
+template hipError_t launch_topn_hist_seeded<float>(hipStream_t, const TopnHistArgs<float>&, uint32_t);
 template hipError_t launch_octave_hist<double>(hipStream_t, const OctaveHistArgs<double>&, uint32_t);
 template hipError_t launch_octave_hist<float>(hipStream_t, const OctaveHistArgs<float>&, uint32_t);
 #define PP_TOPN_INSTANTIATE(W, T)                                                                       \

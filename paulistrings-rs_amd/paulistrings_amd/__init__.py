@@ -291,6 +291,15 @@ class GpuPauliSum:
         """Take ``src``'s terms and layout into this sum's own device memory (pp_sum_copy; asynchronous)."""
         _native.check(_native.lib().pp_sum_copy(self._h, src._h), self.ctx.handle)
 
+    CHECK_NAMES = ("misplaced", "zero", "segments", "duplicates", "unscanned", "overlaps", "length")
+
+    def check_layout(self) -> dict:
+        """The resident layout's invariants (pp_sum_check): violations by kind, all zero for a sound layout
+        ("unscanned" counts buckets too long for the duplicate scan, not a violation)."""
+        out = np.zeros(7, np.uint64)
+        _native.check(_native.lib().pp_sum_check(self._h, out.ctypes.data), self.ctx.handle)
+        return {k: int(v) for k, v in zip(self.CHECK_NAMES, out)}
+
     def bucket_bits(self) -> tuple[int, int]:
         r = ctypes.c_int32()
         d = ctypes.c_int32()
@@ -352,6 +361,24 @@ class GpuPauliSum:
         pol = truncation.lower(policy)
         _native.check(_native.lib().pp_apply_rotation(self._h, ctypes.byref(ch), 1 if adjoint else 0,
                                                       ctypes.byref(pol)), self.ctx.handle)
+
+    def apply_table(self, prepared, policy=None) -> None:
+        """One prepared channel (``tables.unitary_table``, ``kraus_table``, ``rotation_table``, ``table``) as a
+        layer under the policy's per-term filter, no layer pass (pp_apply_prepared)."""
+        pol = truncation.lower(policy)
+        _native.check(_native.lib().pp_apply_prepared(self._h, ctypes.byref(prepared), ctypes.byref(pol)),
+                      self.ctx.handle)
+
+    def apply_run(self, prepared, policy=None, permutations: bool = False) -> list:
+        """A run of prepared channels (rotation tables, or one-to-one steps with ``permutations``) with the
+        length after each (pp_apply_run)."""
+        pol = truncation.lower(policy)
+        arr = (_native.Prepared * len(prepared))(*prepared)
+        lens = np.zeros(len(prepared), np.uint64)
+        kind = _native.RUN_PERMUTATIONS if permutations else _native.RUN_ROTATIONS
+        _native.check(_native.lib().pp_apply_run(self._h, kind, arr, len(prepared), ctypes.byref(pol),
+                                                 lens.ctypes.data), self.ctx.handle)
+        return [int(v) for v in lens]
 
     def apply_rotation_to(self, dst: "GpuPauliSum", channel, adjoint: bool = False, policy=None) -> None:
         """``dst`` <- one rotation layer applied to ``self``, which is left unchanged (asynchronous)."""

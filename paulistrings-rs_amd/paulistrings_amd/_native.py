@@ -122,6 +122,7 @@ def lib() -> ctypes.CDLL:
     L.pp_sum_len.argtypes = [vp, ctypes.POINTER(u64)]
     L.pp_sum_download.argtypes = [vp, vp, vp, vp]
     L.pp_sum_bucket_bits.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.pp_sum_check.argtypes = [vp, vp]
     L.pp_propagate.argtypes = [vp, vp, u64, ctypes.c_int, ctypes.POINTER(Policy), ctypes.POINTER(Options), vp]
     L.pp_apply_rotation.argtypes = [vp, ctypes.POINTER(Channel), ctypes.c_int, ctypes.POINTER(Policy)]
     L.pp_apply_rotation_to.argtypes = [vp, vp, ctypes.POINTER(Channel), ctypes.c_int, ctypes.POINTER(Policy)]
@@ -164,7 +165,7 @@ def lib() -> ctypes.CDLL:
                  "pp_expectation_product", "pp_profile_enable", "pp_profile_read", "pp_profile_read_score",
                  "pp_prepare_channel", "pp_apply_prepared", "pp_apply_run", "pp_finalize_layer", "pp_settle",
                  "pp_sum_refine", "pp_sum_coarsen_to", "pp_stream_probe", "pp_apply_rotation_to_noalloc",
-                 "pp_sum_copy"):
+                 "pp_sum_copy", "pp_sum_check"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

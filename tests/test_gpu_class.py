@@ -1,9 +1,9 @@
 """The class pass (pp_class.hip): a program of local tables applied in one pass, each class bucket's strings held in
 LDS through every layer.
 
-* Rotation runs (three to eight rotation tables): pp_apply_run and pp_propagate give them to the class pass; with
-  PP_ROTATION_RUNS=layers they go layer by layer through the pair join.  Both give the same keys, coefficients and
-  per-layer lengths, and pp_propagate's equal the oracle's.  A sum whose strings crowd one class past the largest LDS
+* Rotation runs (three to eight rotation tables): with PP_ROTATION_RUNS=class pp_apply_run and pp_propagate give them
+  to the class pass; by default they go layer by layer through the pair join.  Both give the same keys, coefficients
+  and per-layer lengths, and pp_propagate's equal the oracle's.  A sum whose strings crowd one class past the largest LDS
   set falls back to the layers one by one, with the same result.
 * General local tables (dense one- and two-qubit transfer matrices of random unitaries, a non-unital channel) through
   pp_apply_prepared: bit for bit against a naive layer that sums each output's contributions in entry order (the
@@ -41,12 +41,15 @@ def check(g, st):
     _native.check(st, g.ctx.handle)
 
 
-class layers_one_by_one:
-    """PP_ROTATION_RUNS=layers for the duration (the library reads it at every run)."""
+class rotation_runs:
+    """PP_ROTATION_RUNS=mode for the duration (the library reads it at every run)."""
+
+    def __init__(self, mode: str):
+        self.mode = mode
 
     def __enter__(self):
         self.saved = os.environ.get("PP_ROTATION_RUNS")
-        os.environ["PP_ROTATION_RUNS"] = "layers"
+        os.environ["PP_ROTATION_RUNS"] = self.mode
 
     def __exit__(self, *exc):
         if self.saved is None:
@@ -82,11 +85,7 @@ def run_vs_layers(host, preps, policy):
         try:
             arr = (_native.Prepared * len(preps))(*preps)
             lens = np.zeros(len(preps), np.uint64)
-            if one_by_one:
-                with layers_one_by_one():
-                    check(g, L.pp_apply_run(g._h, _native.RUN_ROTATIONS, arr, len(preps), ctypes.byref(pol),
-                                            lens.ctypes.data))
-            else:
+            with rotation_runs("layers" if one_by_one else "class"):
                 check(g, L.pp_apply_run(g._h, _native.RUN_ROTATIONS, arr, len(preps), ctypes.byref(pol),
                                         lens.ctypes.data))
             h = g.to_host()
@@ -177,10 +176,7 @@ def test_propagate_with_rotation_runs_equals_layers_and_the_oracle(dtype, direct
         for one_by_one in (False, True):
             g = host.to_device(0)
             try:
-                if one_by_one:
-                    with layers_one_by_one():
-                        st = g.propagate_with_stats(circ, pol, direction)
-                else:
+                with rotation_runs("layers" if one_by_one else "class"):
                     st = g.propagate_with_stats(circ, pol, direction)
                 res.append((list(st.terms_out), g.to_host()))
             finally:

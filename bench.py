@@ -88,6 +88,23 @@ def dense_layer_channels(seed: int):
     return c.channels()
 
 
+def touched_fraction(x, z, channels, sample: int = 1 << 20) -> dict:
+    """Anticommuting rows / N per channel kind, on the first `sample` keys: the share of rows a layer changes."""
+    import numpy as np
+
+    xs, zs = x[:sample], z[:sample]
+    out = {"rzz": [], "rx": []}
+    for i, ch in enumerate(channels):
+        gx = np.asarray(ch["gx"][:2], dtype=np.uint64)
+        gz = np.asarray(ch["gz"][:2], dtype=np.uint64)
+        par = np.zeros(len(xs), dtype=np.uint64)
+        for w in range(2):
+            par ^= (xs[:, w] & gz[w]) ^ (zs[:, w] & gx[w])
+        odd = np.bitwise_count(par) & 1
+        out["rzz" if i < 63 else "rx"].append(float(odd.mean()))
+    return {k: float(np.mean(v)) for k, v in out.items()}
+
+
 def host_sum(x, z, c):
     """A PauliSum over already distinct keys, without the host re-sort."""
     from paulistrings_amd import DEFAULT_HASH_SEED, PauliSum
@@ -743,6 +760,8 @@ def main() -> int:
         log("cpu baseline on one thread")
         cpu1 = cpu_baseline(channels, args.dtype, args.cpu_log2n - 2, args.cpu_seconds / 3, threads=1)
 
+    touched = touched_fraction(x, z, channels)
+    log(f"touched fraction (anticommuting rows / N): RZZ {touched['rzz']:.4f}, RX {touched['rx']:.4f}")
     traffic = None  # from the committed rocprofv3 --pmc summary of this workload (counters need their own run)
     if args.log2n == 24:
         prof = ROOT / "profiles" / f"rotation_inplace_saturated_layer_2e24_{args.dtype}_traffic.md"
@@ -795,6 +814,7 @@ def main() -> int:
             "mean_growth_rzz": float(sum(r["per_class"]["rzz"]) / len(r["per_class"]["rzz"]) / r["n_in"]),
             "mean_growth_rx": float(sum(r["per_class"]["rx"]) / len(r["per_class"]["rx"]) / r["n_in"]),
             "device_bucket_bits": r["device_bits"],
+            "touched_fraction": touched,
             "input_terms_per_s": r["rows_in"] / r["seconds"],
             "speedup_vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
             "cpu_baseline_one_thread": cpu1,

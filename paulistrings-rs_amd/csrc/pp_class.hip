@@ -233,6 +233,14 @@ __global__ void __launch_bounds__(kClassBlock) k_class_apply(const ClassArgs<W, 
                 const T c = IC[i];
                 const uint32_t s = pattern_of<W>(k, s_word[l], s_bit[l], s_mask[l]);
                 const uint16_t* const mem = rec_mem + (size_t)row_rec[i] * 16;
+                // a row alone in its class is every output's only source: one product each, no gather
+                uint32_t members = 0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const uint32_t v = reinterpret_cast<const uint32_t*>(mem)[q];
+                    members += ((v & 0xFFFFu) != 0xFFFFu ? 1u : 0u) + ((v >> 16) != 0xFFFFu ? 1u : 0u);
+                }
+                const bool alone = members == 1u;
                 for (uint32_t e = 0; e < ne; ++e) {
                     if (A[e * 16u + s] == 0.0) continue;
                     const uint32_t t = s ^ D[e];
@@ -240,7 +248,11 @@ __global__ void __launch_bounds__(kClassBlock) k_class_apply(const ClassArgs<W, 
                     xor_local_delta<W>(K, D[e], s_word[l], s_bit[l]);
                     T w = T(0);
                     bool any = false, owner = true;
-                    for (uint32_t f = 0; f < ne; ++f) {
+                    if (alone) {
+                        w = c * (T)A[e * 16u + s];
+                        any = true;
+                    }
+                    for (uint32_t f = 0; f < ne && !alone; ++f) {
                         const double af = A[f * 16u + (t ^ D[f])];
                         if (af == 0.0) continue;
                         T cf = c;

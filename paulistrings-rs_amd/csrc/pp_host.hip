@@ -1196,8 +1196,9 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, cons
 }
 
 // Whether a join layer runs in place (k_rotation_inplace) rather than into the spare columns: the layout's segment
-// rooms are known, no TopN selection is pending (every row it drops would be a hole to fill), the layer is not a
-// Clifford arm (every anticommuting row would be one) and earlier moves have not left more dead rows than live ones.
+// rooms are known, the layer is not a Clifford arm (every anticommuting row would leave a hole) and earlier moves have
+// not left more dead rows than live ones.  A pending TopN selection is read through, its dropped rows holes that the
+// new rows and the moved-down rows above each side's final length fill.
 // Below 2^18 rows the layer is launch-bound and the out-of-place join, one synchronization, is the faster
 // (PP_IN_PLACE_MIN_ROWS overrides the bound, for tests of the in-place path on small sums).
 uint64_t in_place_min_rows() {
@@ -1205,9 +1206,15 @@ uint64_t in_place_min_rows() {
     return e ? std::strtoull(e, nullptr, 10) : (1ull << 18);
 }
 
+// Whether a side of an in-place layer that loses more rows than it gains closes its holes in place (default) or
+// sends its pair to be moved whole to fresh segments (PP_IN_PLACE_COMPACT=0, for comparison: the same result).
+bool in_place_compaction() {
+    const char* e = std::getenv("PP_IN_PLACE_COMPACT");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
 bool join_in_place(const pp_sum* s, const Prepared& p) {
-    if (!s->ctx->tuning.join_in_place || !s->has_room || s->pending_topn || p.clifford || s->len == 0 ||
-        s->len < in_place_min_rows())
+    if (!s->ctx->tuning.join_in_place || !s->has_room || p.clifford || s->len == 0 || s->len < in_place_min_rows())
         return false;
     const uint64_t B = 1ull << s->device_bits;
     return s->tail <= 3 * s->len + 4ull * kSegAlign * B;
@@ -1311,6 +1318,12 @@ pp_status layer_in_place(pp_sum* s, const Prepared& p, const KeepProgram& keep) 
             a.redo_count = s->d_stats + 4;
             a.redo_mode = redo_mode;
             a.count_totals = count_in_join ? 1u : 0u;
+            a.compact = in_place_compaction() ? 1u : 0u;
+            if (s->pending_topn) {  // the layer reads its input through the selection
+                a.filter = s->d_topn;
+                a.filter_flags = s->pending_flags;
+                a.filter_score = s->pending_score;
+            }
             return launch_rotation_inplace<W, T>(ctx->stream, a);
         });
     };
@@ -1348,6 +1361,7 @@ pp_status layer_in_place(pp_sum* s, const Prepared& p, const KeepProgram& keep) 
     s->len = s->h_stats[0];
     s->max_bucket = (uint32_t)s->h_stats[1];
     s->tail += s->h_stats[5];
+    s->pending_topn = false;  // the layer read its input through the selection
     if (timed) ctx->prof_rows_out += s->len;
     return PP_OK;
 }

@@ -83,6 +83,11 @@ struct InplaceArgs {
     unsigned long long* redo_count;
     uint32_t redo_mode;
     uint32_t count_totals;
+    uint32_t compact;  // a side left with more holes than new rows closes them in place (else the pair moves)
+    // a TopN selection the layer reads its input through (the filtered launch): rows it drops are holes
+    const TopnState* filter;
+    const uint8_t* filter_flags;
+    const float* filter_score;
 };
 
 // A local table applied row by row, one output per row (a permutation step, PermutationStep::of,

@@ -567,8 +567,10 @@ __global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a)
 // rows, which go first into the rows the layer drops on their side (holes) and then after the side's last row.  A
 // commuting row that stays is not written.  Every decision is taken before the first write.  The first launch
 // leaves a pair untouched and queues it when it stages more anticommuting rows than the workgroup holds or does not
-// fit its segments (a side with more holes than new rows, or more rows than its segment's room), adding the rows its
-// move may take to counter 7.  The redo launch (REDO) stages the largest pair: a queued pair that fits is done in
+// fit its segments (more rows than a segment's room), adding the rows its move may take to counter 7.  A side that
+// loses more rows than it gains closes its remaining holes in place: the new rows take its lowest holes, and the rows
+// that stay above its final length move down into the others (their order does not matter in a bucket).  With
+// FILTER the input carries a pending TopN selection: a row the selection drops is a hole.  The redo launch (REDO) stages the largest pair: a queued pair that fits is done in
 // place, any other is written whole to fresh segments after the columns' last used row (tail_base plus counter 5,
 // which the host has made room for), its old segments left as dead space.
 // ---------------------------------------------------------------------------------------------------------------
@@ -586,11 +588,42 @@ size_t inplace_lds_bytes(uint32_t scap, uint32_t tcap_bits) {
            2 * (size_t)kInplaceHoles * 2;
 }
 
-template <int W, typename T, bool REDO>
+// A side's holes as a bitmap of its rows (kInplaceHoles bits) with the holes before each word (pre[w], pre[32] all):
+// the k-th hole in ascending order, and the holes below row p, each in a few steps.
+constexpr uint32_t kHoleWords = kInplaceHoles / 32;
+__device__ __forceinline__ uint32_t holes_below(const uint32_t* bits, const uint16_t* pre, uint32_t p) {
+    const uint32_t w = p >> 5, b = p & 31u;
+    return pre[w] + (b ? (uint32_t)__popc(bits[w] & ((1u << b) - 1u)) : 0u);
+}
+
+__device__ __forceinline__ uint32_t kth_hole(const uint32_t* bits, const uint16_t* pre, uint32_t k) {
+    uint32_t lo = 0, hi = kHoleWords;  // the word w with pre[w] <= k < pre[w + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    uint32_t v = bits[lo], pos = 0;
+    k -= pre[lo];
+#pragma unroll
+    for (uint32_t half = 16; half >= 1; half >>= 1) {  // select the k-th set bit by halves
+        const uint32_t c = (uint32_t)__popc(v & ((1u << half) - 1u));
+        if (k >= c) {
+            k -= c;
+            v >>= half;
+            pos += half;
+        }
+    }
+    return lo * 32u + pos;
+}
+
+template <int W, typename T, bool REDO, bool FILTER>
 __global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W, T> a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t scan_scratch[4 * kWaves];
     __shared__ uint64_t reloc_base;
+    __shared__ uint32_t hbits[2][kHoleWords];
+    __shared__ uint16_t hpre[2][kHoleWords + 1];
     const uint32_t S = a.scap;
     uint64_t* const Lx = reinterpret_cast<uint64_t*>(lds);
     uint64_t* const Lz = Lx + (size_t)W * S;
@@ -615,6 +648,17 @@ __global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W
     }
     const uint32_t supp = support_words<W>(g);
     const uint32_t fold_g = key_fold<W>(g);
+    TopnState fs{};
+    if constexpr (FILTER) fs = *a.filter;
+    // whether the TopN selection keeps the row at column position i (coefficient c)
+    auto selected = [&](uint64_t i, T c) -> bool {
+        if constexpr (!FILTER) {
+            return true;
+        } else {
+            const uint64_t u = a.filter_score ? score_bits(a.filter_score[i]) : magnitude_bits<T>(c);
+            return topn_keeps(fs, u, !a.filter_score && is_nan_bits(u), a.filter_flags, i);
+        }
+    };
     for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {  // uniform
         const uint32_t p = REDO ? a.redo[t] : t;
         uint32_t b0, b1;
@@ -633,6 +677,9 @@ __global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W
         const uint32_t tmask = (1u << tbits) - 1u;
         __syncthreads();  // the previous pair's LDS is no longer read
         for (uint32_t i = threadIdx.x; i < (1u << tbits); i += kBlock) table[i] = 0u;
+        if (threadIdx.x < 2 * kHoleWords) (&hbits[0][0])[threadIdx.x] = 0u;
+        // a side of more rows than the bitmap covers cannot close its holes in place
+        const bool compactable = a.compact && n0 <= kInplaceHoles && n1 <= kInplaceHoles;
         __syncthreads();
         // ---- phase A: holes and staging; no global write ----
         uint32_t hc0 = 0, hc1 = 0, nl = 0;
@@ -645,17 +692,21 @@ __global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W
             const uint64_t row = j ? s1 + idx : s0 + idx;
             Key<W> k;
             T c = T(0);
-            const bool valid = in;
+            bool valid = in;
             if (in) {
                 k = load_key<W>(a.x, a.z, a.cap, row);
                 c = a.c[row];
+                valid = selected(row, c);
             }
             const bool anti = valid && anticommutes_on<W>(k, g, supp);
             const bool hole = in && !anti && !(valid && survives<W, T>(a.keep, k, c));
             bool f[4] = {hole && j == 0, hole && j == 1, anti && j == 0, anti && j == 1};
             uint32_t off[4], tot[4];
             block_flag_scan<4>(f, off, tot, scan_scratch);
-            if (hole) holes[j * kInplaceHoles + (j ? hc1 + off[1] : hc0 + off[0])] = (uint16_t)idx;
+            if (hole) {
+                holes[j * kInplaceHoles + (j ? hc1 + off[1] : hc0 + off[0])] = (uint16_t)idx;
+                if (compactable) atomicOr(&hbits[j][idx >> 5], 1u << (idx & 31u));
+            }
             overflow = overflow || nl + tot[2] + tot[3] > S;
             if (anti && !overflow) {
                 const uint32_t li = nl + off[2] + off[3];
@@ -744,7 +795,10 @@ __global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W
             bool f[4] = {drop && jj == 0, drop && jj == 1, keep_new && jn == 0, keep_new && jn == 1};
             uint32_t off[4], tot[4];
             block_flag_scan<4>(f, off, tot, scan_scratch);
-            if (drop) holes[jj * kInplaceHoles + (jj ? hc1 + off[1] : hc0 + off[0])] = Li[i];
+            if (drop) {
+                holes[jj * kInplaceHoles + (jj ? hc1 + off[1] : hc0 + off[0])] = Li[i];
+                if (compactable) atomicOr(&hbits[jj][Li[i] >> 5], 1u << (Li[i] & 31u));
+            }
             if (valid) {
                 Lflag[i] = (uint8_t)((keep_own ? 1u : 0u) | (keep_new ? 2u : 0u));
                 Lrank[i] = (uint16_t)(jn ? nc1 + off[3] : nc0 + off[2]);
@@ -756,7 +810,8 @@ __global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W
         }
         // rows that stay, per side: every row of the side but its holes
         const uint32_t fin0 = n0 - hc0 + nc0, fin1 = n1 - hc1 + nc1;
-        const bool fits = hc0 <= nc0 && fin0 <= cap0 && (!pair || (hc1 <= nc1 && fin1 <= cap1));
+        const bool fits = fin0 <= cap0 && (!pair || fin1 <= cap1) &&
+                          (compactable || (hc0 <= nc0 && (!pair || hc1 <= nc1)));
         __syncthreads();  // every decision and hole in LDS
         if (!fits && !REDO) {
             if (threadIdx.x == 0) {
@@ -766,16 +821,52 @@ __global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W
             continue;
         }
         if (fits) {
-            // ---- phase C in place: kept own coefficients, new rows into the holes and then after the rows ----
+            // ---- phase C in place: kept own coefficients, new rows into the holes and then after the rows; a
+            // side left with holes (more holes than new rows) takes its new rows into its lowest holes and moves the
+            // rows that stay above its final length into the holes left below it ----
+            const bool shrink0 = hc0 > nc0, shrink1 = pair && hc1 > nc1;
+            if (shrink0 || shrink1) {  // uniform: the holes before each word of the bitmap
+                if (threadIdx.x < 2) {
+                    uint32_t acc = 0;
+                    for (uint32_t w = 0; w < kHoleWords; ++w) {
+                        hpre[threadIdx.x][w] = (uint16_t)acc;
+                        acc += (uint32_t)__popc(hbits[threadIdx.x][w]);
+                    }
+                    hpre[threadIdx.x][kHoleWords] = (uint16_t)acc;
+                }
+                __syncthreads();
+            }
             for (uint32_t i = threadIdx.x; i < nl; i += kBlock) {
                 const uint32_t fl = Lflag[i];
                 const uint32_t jj = Lm[i];
-                if (fl & 1u) a.c[(jj ? s1 : s0) + Li[i]] = Lown[i];
+                if (fl & 1u) {
+                    const uint32_t at = Li[i];
+                    const uint32_t fin = jj ? fin1 : fin0;
+                    if ((jj ? shrink1 : shrink0) && at >= fin) {  // a staying row above the final length moves down
+                        // its rank among the rows that stay above the final length, after the new rows
+                        const uint32_t above = (at - fin) - (holes_below(hbits[jj], hpre[jj], at) -
+                                                             holes_below(hbits[jj], hpre[jj], fin));
+                        const uint32_t to = kth_hole(hbits[jj], hpre[jj], (jj ? nc1 : nc0) + above);
+                        const uint64_t pos = (jj ? s1 : s0) + to;
+                        Key<W> kk;
+#pragma unroll
+                        for (int w = 0; w < W; ++w) {
+                            kk.x[w] = Lx[w * S + i];
+                            kk.z[w] = Lz[w * S + i];
+                        }
+                        store_key<W>(a.x, a.z, a.cap, pos, kk);
+                        store_coeff<T>(a.c, pos, Lown[i]);
+                    } else {
+                        a.c[(jj ? s1 : s0) + at] = Lown[i];
+                    }
+                }
                 if (fl & 2u) {
                     const uint32_t jn = pair ? (jj ^ 1u) : jj;
                     const uint32_t rk = Lrank[i];
                     const uint32_t hcn = jn ? hc1 : hc0;
-                    const uint32_t slot = rk < hcn ? holes[jn * kInplaceHoles + rk] : (jn ? n1 : n0) + (rk - hcn);
+                    const uint32_t slot = (jn ? shrink1 : shrink0) ? kth_hole(hbits[jn], hpre[jn], rk)
+                                          : rk < hcn ? holes[jn * kInplaceHoles + rk]
+                                                     : (jn ? n1 : n0) + (rk - hcn);
                     const uint64_t pos = (jn ? s1 : s0) + slot;
                     Key<W> q;
 #pragma unroll
@@ -785,6 +876,24 @@ __global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W
                     }
                     store_key<W>(a.x, a.z, a.cap, pos, q);
                     store_coeff<T>(a.c, pos, Lfresh[i]);
+                }
+            }
+            // commuting rows that stay above a shrinking side's final length: read here (nothing writes at or above
+            // the final length) and moved into the holes after the new rows'
+            for (int jj = 0; jj < (pair ? 2 : 1); ++jj) {
+                if (!(jj ? shrink1 : shrink0)) continue;  // uniform
+                const uint32_t fin = jj ? fin1 : fin0, nn = jj ? n1 : n0;
+                const uint64_t sj = jj ? s1 : s0;
+                for (uint32_t at = fin + threadIdx.x; at < nn; at += kBlock) {
+                    if ((hbits[jj][at >> 5] >> (at & 31u)) & 1u) continue;
+                    const Key<W> k = load_key<W>(a.x, a.z, a.cap, sj + at);
+                    if (anticommutes_on<W>(k, g, supp)) continue;  // a staged row: moved above
+                    const T c = a.c[sj + at];
+                    const uint32_t above = (at - fin) - (holes_below(hbits[jj], hpre[jj], at) -
+                                                         holes_below(hbits[jj], hpre[jj], fin));
+                    const uint32_t to = kth_hole(hbits[jj], hpre[jj], (jj ? nc1 : nc0) + above);
+                    store_key<W>(a.x, a.z, a.cap, sj + to, k);
+                    store_coeff<T>(a.c, sj + to, c);
                 }
             }
             if (threadIdx.x == 0) {
@@ -822,10 +931,11 @@ __global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W
             const uint64_t row = j ? s1 + (r - n0) : s0 + r;
             Key<W> k;
             T c = T(0);
-            const bool valid = in;
+            bool valid = in;
             if (in) {
                 k = load_key<W>(a.x, a.z, a.cap, row);
                 c = a.c[row];
+                valid = selected(row, c);
             }
             const bool stay = valid && !anticommutes_on<W>(k, g, supp) && survives<W, T>(a.keep, k, c);
             bool f[2] = {stay && j == 0, stay && j == 1};
@@ -1342,8 +1452,12 @@ hipError_t launch_rotation_inplace(hipStream_t st, const InplaceArgs<W, T>& a) {
     if (a.num_cosets == 0) return hipSuccess;
     if (a.scap > kPairRowsMax || a.tcap_bits > 11) return hipErrorInvalidValue;
     const size_t lds = inplace_lds_bytes<W, T>(a.scap, a.tcap_bits);
+    const bool filter = a.filter != nullptr;
     if (!a.redo_mode) {
-        hipLaunchKernelGGL((k_rotation_inplace<W, T, false>), dim3(a.num_cosets), dim3(kBlock), lds, st, a);
+        if (filter)
+            hipLaunchKernelGGL((k_rotation_inplace<W, T, false, true>), dim3(a.num_cosets), dim3(kBlock), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_rotation_inplace<W, T, false, false>), dim3(a.num_cosets), dim3(kBlock), lds, st, a);
         return hipGetLastError();
     }
     static int cus = 0;
@@ -1354,11 +1468,15 @@ hipError_t launch_rotation_inplace(hipStream_t st, const InplaceArgs<W, T>& a) {
             cus = 256;
     }
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rotation_inplace<W, T, true>, kBlock, lds) !=
-            hipSuccess ||
-        per <= 0)
-        per = 1;
-    hipLaunchKernelGGL((k_rotation_inplace<W, T, true>), dim3(cus * per), dim3(kBlock), lds, st, a);
+    const hipError_t oe = filter ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                       &per, k_rotation_inplace<W, T, true, true>, kBlock, lds)
+                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                       &per, k_rotation_inplace<W, T, true, false>, kBlock, lds);
+    if (oe != hipSuccess || per <= 0) per = 1;
+    if (filter)
+        hipLaunchKernelGGL((k_rotation_inplace<W, T, true, true>), dim3(cus * per), dim3(kBlock), lds, st, a);
+    else
+        hipLaunchKernelGGL((k_rotation_inplace<W, T, true, false>), dim3(cus * per), dim3(kBlock), lds, st, a);
     return hipGetLastError();
 }
 

@@ -3164,6 +3164,15 @@ pp_status pp_batch_propagate(pp_batch* b, const pp_channel* channels, const uint
             pp_sum* s = b->sums[i];
             if (next[i] == run_end[i]) run_end[i] = plan.start_run(s, next[i]);
             const pp_channel& ch = plan.channel_at(next[i]);
+            if (run_end[i] - next[i] >= 2 && s->len > 0 && fused_permutation_runs() &&
+                is_permutation_step(prepare(s, ch, plan.adjoint))) {
+                // a permutation run: walked in one pass for this instance (RunPlanner::apply_run), not step by step
+                // in the batched join
+                if ((st = plan.apply_run(s, next[i], run_end[i], keeps[i], nullptr)) != PP_OK) return st;
+                next[i] = run_end[i];
+                if ((st = plan.finish_run(s, next[i] < plan.n_channels)) != PP_OK) return st;
+                continue;
+            }
             if ((st = settle_device_bits(s)) != PP_OK) return st;
             const Prepared p = prepare(s, ch, plan.adjoint);
             if (p.key_preserving) {  // in place, on its own

@@ -684,20 +684,30 @@ __global__ void __launch_bounds__(kBlock) k_rotation_inplace(const InplaceArgs<W
         // ---- phase A: holes and staging; no global write ----
         uint32_t hc0 = 0, hc1 = 0, nl = 0;
         bool overflow = false;
+        // each thread's row of a chunk, loaded one chunk ahead: the next chunk's loads are in flight while this
+        // chunk's scan and staging run
+        auto fetch = [&](uint32_t base, Key<W>& k, T& c) {
+            const uint32_t r = base + threadIdx.x;
+            if (r < n) {
+                const uint64_t row = r >= n0 ? s1 + (r - n0) : s0 + r;
+                k = load_key<W>(a.x, a.z, a.cap, row);
+                c = a.c[row];
+            }
+        };
+        Key<W> kn{};
+        T cn = T(0);
+        fetch(0, kn, cn);
         for (uint32_t base = 0; base < n; base += kBlock) {
             const uint32_t r = base + threadIdx.x;
             const bool in = r < n;
             const uint32_t j = r >= n0 ? 1u : 0u;
             const uint32_t idx = j ? r - n0 : r;
             const uint64_t row = j ? s1 + idx : s0 + idx;
-            Key<W> k;
-            T c = T(0);
+            const Key<W> k = kn;
+            const T c = cn;
+            if (base + kBlock < n) fetch(base + kBlock, kn, cn);  // uniform
             bool valid = in;
-            if (in) {
-                k = load_key<W>(a.x, a.z, a.cap, row);
-                c = a.c[row];
-                valid = selected(row, c);
-            }
+            if (in) valid = selected(row, c);
             const bool anti = valid && anticommutes_on<W>(k, g, supp);
             const bool hole = in && !anti && !(valid && survives<W, T>(a.keep, k, c));
             bool f[4] = {hole && j == 0, hole && j == 1, anti && j == 0, anti && j == 1};

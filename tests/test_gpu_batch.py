@@ -60,6 +60,23 @@ def test_batched_instances_equal_their_solo_runs():
     check_batch(inst, pol_o, pol_p)
 
 
+def test_batched_permutation_runs_walked_per_instance():
+    """Instances whose circuits carry runs of quarter-turn RZZ (permutation runs, walked in one pass per instance
+    inside the batch) between their brick-wall layers, one of them under TopN: each equals its solo oracle run."""
+    announce("batch with permutation runs")
+    inst = circuits.batched_instances(seed=19, count=10, depths=(3, 6), thresholds=(3e-4, 1e-2))
+    for k, (n, circ, obs, eps) in enumerate(inst):
+        if k % 2 == 0:
+            for q in range(0, n - 1, 3):
+                circ.pauli_rotation("ZZ", [q, q + 1], -math.pi / 2)
+            circ.rx(0.3, 1)
+    pol_o = [po.coeff_policy(eps=eps, topn=200) if k == 4 else po.coeff_policy(eps=eps)
+             for k, (_n, _c, _o, eps) in enumerate(inst)]
+    pol_p = [truncation.coeff(eps) & truncation.topn(200) if k == 4 else truncation.coeff(eps)
+             for k, (_n, _c, _o, eps) in enumerate(inst)]
+    check_batch(inst, pol_o, pol_p)
+
+
 def test_batch_of_wide_sums_forward():
     announce("batch of 3 sums on 100-127 qubits, forward")
     rng = np.random.default_rng(12)

@@ -219,14 +219,14 @@ def stream_probe(bytes_per_array: int = 1 << 30, reps: int = 5) -> dict:
     return {"read": out[0], "write": out[1], "copy": out[2], "triad": out[3]}
 
 
-def summarize(r: dict, dtype_name: str, steps: int) -> dict:
+def summarize(r: dict, dtype_name: str, steps: int, layers_per_step: int) -> dict:
     """One timed measurement as value, ms per step and the join's algorithmic rate."""
     bytes_per_row = 2 * 2 * 8 + (8 if dtype_name == "f64" else 4)
     kernel_s = r["kernel_ms"] / 1e3
     achieved = (r["rows_in"] + r["rows_out"]) * bytes_per_row / kernel_s / 1e9 if kernel_s > 0 else 0.0
     return {"dtype": dtype_name, "value": r["rows_out"] / r["seconds"], "unit": "terms/s",
-            "ms_per_step": r["seconds"] / steps * 1e3, "kernel_avg_ms": r["kernel_ms"] / max(1, r["launches"]),
-            "achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_row": bytes_per_row}
+            "ms_per_step": r["seconds"] / steps * 1e3, "kernel_ms_per_layer": r["kernel_ms"] / (steps * layers_per_step),
+            "kernel_launches": r["launches"], "achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_row": bytes_per_row}
 
 
 def cpu_model() -> str:
@@ -736,20 +736,22 @@ def main() -> int:
     stream = {}
     if not args.main_only:
         log("the same layers out of place (into a second sum, queued back to back)")
-        oop = summarize(gpu_measure(x, z, c, channels, args.steps, args.warmup), args.dtype, args.steps)
+        oop = summarize(gpu_measure(x, z, c, channels, args.steps, args.warmup), args.dtype, args.steps,
+                        len(channels))
         log(f"out of place: {oop['value']:.3e} merged terms/s, join kernel {oop['achieved']:.0f} GB/s")
         stream = stream_probe()
         log("STREAM probe (GB/s): " + ", ".join(f"{k} {v:.0f}" for k, v in stream.items()))
     if args.dtype == "f64" and not args.main_only:  # the same keys, coefficients rounded to float32
         log(f"float32 coefficients: timed {args.steps} steps of {len(channels)} layers")
         other = summarize(gpu_measure_in_place(x, z, c.astype(np.float32), channels, args.steps, args.warmup),
-                          "f32", args.steps)
+                          "f32", args.steps, len(channels))
         log(f"float32: {other['value']:.3e} merged terms/s, join kernel {other['achieved']:.0f} GB/s")
     bytes_per_row = 2 * 2 * 8 + (8 if args.dtype == "f64" else 4)
     alg_bytes = (r["rows_in"] + r["rows_out"]) * bytes_per_row
     kernel_s = r["kernel_ms"] / 1e3
     achieved = alg_bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
     value = r["rows_out"] / r["seconds"]
+    layers = args.steps * len(channels)
     log(f"{value:.3e} merged terms/s, join kernel {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
 
     cpu = None
@@ -805,11 +807,14 @@ def main() -> int:
                         "writes only the coefficients it changes and the new rows, so achieved (algorithmic bytes of "
                         "an out-of-place layer over its time) can exceed the HBM rate it moves: see traffic",
             "out_of_place": oop,
+            "layers": layers,
             "kernel_launches": r["launches"],
-            "kernel_avg_ms": r["kernel_ms"] / max(1, r["launches"]),
+            "launch_unit": "one layer: the first k_rotation_inplace launch, plus a second over queued pairs on the "
+                           "layers where a pair outgrew the first launch's stage",
+            "kernel_ms_per_layer": r["kernel_ms"] / max(1, layers),
             "kernel_share_of_wall": kernel_s / r["seconds"],
-            "rows_in_per_launch": r["rows_in"] / max(1, r["launches"]),
-            "rows_out_per_launch": r["rows_out"] / max(1, r["launches"]),
+            "rows_in_per_layer": r["rows_in"] / max(1, layers),
+            "rows_out_per_layer": r["rows_out"] / max(1, layers),
             "algorithmic_bytes_per_row": bytes_per_row,
             "mean_growth_rzz": float(sum(r["per_class"]["rzz"]) / len(r["per_class"]["rzz"]) / r["n_in"]),
             "mean_growth_rx": float(sum(r["per_class"]["rx"]) / len(r["per_class"]["rx"]) / r["n_in"]),

@@ -95,12 +95,15 @@ def test_synchronizing_calls_are_refused_during_capture():
     g.free()
 
 
-def test_replay_is_refused_once_the_captured_buffers_changed():
+def test_replay_is_refused_once_the_captured_buffers_changed(monkeypatch):
     """A replay writes through the buffers the capture baked in.  After a layer in place on its output (same
     columns, same bucket arrays) a replay is accepted and rewrites the output whole; once the output's current
     column set flipped (a quarter turn runs out of place) the launch is refused on the host, before anything is
-    queued, and a sum a live graph uses cannot be freed.  The refusal needs no kernel: the check runs first."""
+    queued, and a sum a live graph uses cannot be freed.  The refusal needs no kernel: the check runs first.  The
+    sum is small, so the in-place layer is switched on for it (below 2^18 rows a layer otherwise runs out of
+    place)."""
     announce("graph replay refused after an out-of-place layer on its output")
+    monkeypatch.setenv("PP_IN_PLACE_MIN_ROWS", "0")
     from paulistrings_amd import _native
 
     rng = np.random.default_rng(43)

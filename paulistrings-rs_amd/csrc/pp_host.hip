@@ -39,6 +39,8 @@ constexpr int kRunDimension = 3;                  // engine/bucketed/rotation_la
 constexpr uint32_t kMaxPairRows = 1024;           // the largest pair a join's redo launch stages
 // Device counters of a sum: [0] rows, [1] longest bucket, [2] join overflow, [3] offsets total, [4] redo pairs.
 constexpr int kStatSlots = 8;
+// The sum's pinned copies of them: one for a synchronous read, two for the counters of layers queued ahead of them.
+constexpr int kPinnedStatSlots = 3 * kStatSlots;
 
 // Defaults of the per-context tuning, each overridable from the environment.
 // The join's grid (PP_JOIN_GROUPS_PER_CU): one workgroup per pair by default (measured fastest on the saturated
@@ -281,6 +283,7 @@ struct pp_context {
     void* stage[2] = {nullptr, nullptr};   // pinned host chunks of the bulk transfers, allocated on first use
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     void* class_pinned = nullptr;  // a class pass's program on its way up and its counters on their way down
+    hipEvent_t lag_ev[2] = {nullptr, nullptr};  // the counters of a layer queued ahead of them have arrived
 };
 
 struct pp_graph {
@@ -2049,6 +2052,8 @@ void pp_context_destroy(pp_context* ctx) {
         if (ctx->stage_ev[k]) (void)hipEventDestroy(ctx->stage_ev[k]);
     }
     if (ctx->class_pinned) (void)hipHostFree(ctx->class_pinned);
+    for (hipEvent_t e : ctx->lag_ev)
+        if (e) (void)hipEventDestroy(e);
     delete ctx;
 }
 
@@ -2105,7 +2110,8 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
     pp_status st;
     if ((st = dev_alloc(ctx, (void**)&s->d_rows, s->rows.size() * 8)) != PP_OK) return bail(st);
     if ((st = dev_alloc(ctx, (void**)&s->d_stats, kStatSlots * sizeof(unsigned long long))) != PP_OK) return bail(st);
-    if (hipHostMalloc((void**)&s->h_stats, kStatSlots * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&s->h_stats, kPinnedStatSlots * sizeof(unsigned long long), hipHostMallocDefault) !=
+        hipSuccess)
         return bail(fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned host allocation"));
     if (hipMemcpy(s->d_rows, s->rows.data(), s->rows.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(ctx, PP_ERR_DRIVER, "hash rows upload"));
@@ -2254,7 +2260,8 @@ pp_status pp_sum_clone(const pp_sum* src_c, pp_sum** out) {
     pp_status st;
     if ((st = dev_alloc(ctx, (void**)&s->d_rows, s->rows.size() * 8)) != PP_OK) return bail(st);
     if ((st = dev_alloc(ctx, (void**)&s->d_stats, kStatSlots * sizeof(unsigned long long))) != PP_OK) return bail(st);
-    if (hipHostMalloc((void**)&s->h_stats, kStatSlots * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&s->h_stats, kPinnedStatSlots * sizeof(unsigned long long), hipHostMallocDefault) !=
+        hipSuccess)
         return bail(fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned host allocation"));
     const Columns& sc = src->col[src->cur];
     const BucketArrays& sb = src->bk[src->cur];
@@ -2735,6 +2742,138 @@ pp_status pp_sum_coarsen_to(pp_sum* s, int32_t bits) {
     return PP_OK;
 }
 
+namespace {
+
+// ---- lagged lengths (engine/gpu/length.rs:1-4) ------------------------------------------------------------------
+//
+// A small rotation layer's exact length and longest bucket reach the host one layer late.  Each queued layer's
+// counters are copied to a pinned slot behind its kernels; the host waits for the previous layer's copy while the
+// current layer runs, so the GPU is not left idle across a host round trip per layer.  The next layer is sized from
+// bounds: a rotation layer at most doubles the rows, and an output bucket draws on two input buckets, so at most
+// doubles the longest bucket.  A layer whose bounds do not fit (a pair past the largest a join stages, a column set
+// that must grow), a sum large enough for the in-place layer, or a table that is not a rotation drains the queue and
+// takes the synchronous path.  The results are the same either way: only when the host learns the counts changes.
+
+constexpr uint64_t kLagMaxRows = 1ull << 20;  // above this a layer is bandwidth-bound and its synchronization noise
+constexpr uint32_t kLagDepth = 1;             // layers whose counts may still be on their way when one is queued
+
+// Whether pp_propagate queues small layers ahead of their counts (default) or reads every layer's counts before the
+// next (PP_LAGGED_LENGTHS=0, for comparison: the same keys, coefficients and per-layer lengths).
+bool lagged_lengths() {
+    const char* e = std::getenv("PP_LAGGED_LENGTHS");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
+// run_layers under a policy with no layer pass, its rotation layers queued ahead of their counts.  The planner's
+// bookkeeping (the reference's bucket count at every run start) runs as each length arrives.
+pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram& keep, uint64_t* layer_len) {
+    pp_context* ctx = s->ctx;
+    pp_status st;
+    for (hipEvent_t& e : ctx->lag_ev)
+        if (!e) PP_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const uint64_t n = plan.n_channels;
+    const uint64_t target = ctx->tuning.bucket_target;
+    uint64_t i = 0, run_end = 0;  // layer i is the next to queue; the planner's current run ends at run_end
+    uint32_t queued = 0, head = 0;  // layers whose counters are on their way; ring slot of the oldest
+    uint64_t queued_layer[2] = {};
+    uint64_t known_len = s->len, known_mb = s->max_bucket;  // counts after the last layer read
+    // the planner's bookkeeping at layer r, once the length before it is known
+    auto plan_at = [&](uint64_t r, uint64_t len_before) {
+        if (r != run_end || r >= n) return;
+        const uint64_t len_now = s->len;
+        s->len = len_before;
+        run_end = plan.start_run(s, r);
+        s->len = len_now;
+    };
+    auto read_oldest = [&]() -> pp_status {
+        const uint32_t k = head;
+        PP_HIP(ctx, hipEventSynchronize(ctx->lag_ev[k]));
+        const unsigned long long* h = s->h_stats + kStatSlots * (1 + k);
+        if (h[2]) return fail(ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
+        known_len = h[0];
+        known_mb = h[1];
+        s->tail = h[3];
+        head ^= 1u;
+        --queued;
+        if (layer_len) layer_len[queued_layer[k]] = known_len;
+        plan_at(queued_layer[k] + 1, known_len);
+        return PP_OK;
+    };
+    auto drain = [&]() -> pp_status {
+        while (queued) {
+            pp_status rs = read_oldest();
+            if (rs != PP_OK) return rs;
+        }
+        s->len = known_len;
+        s->max_bucket = (uint32_t)known_mb;
+        return PP_OK;
+    };
+    while (i < n) {
+        if (queued == 0 && i == run_end) run_end = plan.start_run(s, i);
+        const Prepared p = prepare(s, plan.channel_at(i), plan.adjoint);
+        bool lag = p.rotation && !p.key_preserving;
+        if (lag && is_permutation_step(p) && i + 1 < n &&
+            is_permutation_step(prepare(s, plan.channel_at(i + 1), plan.adjoint)))
+            lag = false;  // the start of a permutation run: walked in one pass
+        if (lag && queued == 0) {
+            // exact counts: the device bucketing for the layers to come, fine enough that a pair stays within the
+            // join's largest stage under the doubled bound (but not so fine that the next settle coarsens it back)
+            if ((st = settle_device_bits(s)) != PP_OK) return st;
+            while (4ull * s->max_bucket > kMaxPairRows && s->device_bits < s->max_bits &&
+                   s->len * 8 >= target << (s->device_bits + 1))
+                if ((st = refine_once(s)) != PP_OK) return st;
+            known_len = s->len;
+            known_mb = s->max_bucket;
+        }
+        const uint64_t len_hi = known_len << queued, mb_hi = known_mb << queued;
+        lag = lag && len_hi <= kLagMaxRows && 2 * mb_hi <= kMaxPairRows &&
+              (!ctx->tuning.join_in_place || len_hi < in_place_min_rows());
+        const int out = 1 - s->cur;
+        const uint32_t B = 1u << s->device_bits;
+        const uint64_t need = 2 * len_hi + (uint64_t)kSegAlign * B;
+        if (lag && (s->col[out].cap < need || !s->col[out].x || s->bk[out].cap < B || !s->bk[out].start)) {
+            // the output columns must grow: not under a queued layer, which may still read them
+            if (queued) {
+                if ((st = drain()) != PP_OK) return st;
+                continue;
+            }
+            if ((st = ensure_columns(s, out, 4 * need)) != PP_OK) return st;  // room for the layers that follow
+            if ((st = ensure_buckets(s, out, B)) != PP_OK) return st;
+        }
+        if (!lag) {
+            if ((st = drain()) != PP_OK) return st;  // the planner has now seen every length before layer i
+            if (i == run_end) run_end = plan.start_run(s, i);
+            const uint64_t end = run_end;
+            if ((st = plan.apply_run(s, i, end, keep, layer_len ? layer_len + i : nullptr)) != PP_OK) return st;
+            if ((st = plan.finish_run(s, end < n)) != PP_OK) return st;
+            if ((st = debug_check(s)) != PP_OK) return st;
+            known_len = s->len;
+            known_mb = s->max_bucket;
+            i = end;
+            continue;
+        }
+        // queue layer i, sized from the bounds; its counters follow it to the next ring slot
+        s->len = len_hi;
+        s->max_bucket = (uint32_t)mb_hi;
+        if ((st = launch_join(s, s, out, p, keep)) != PP_OK) return st;
+        const uint32_t k = (head + queued) & 1u;
+        PP_HIP(ctx, hipMemcpyAsync(s->h_stats + kStatSlots * (1 + k), s->d_stats,
+                                   kStatSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(ctx, hipEventRecord(ctx->lag_ev[k], ctx->stream));
+        queued_layer[k] = i;
+        ++queued;
+        s->cur = out;
+        s->has_room = true;
+        s->pending_topn = false;
+        ++i;
+        if (queued > kLagDepth && (st = read_oldest()) != PP_OK) return st;
+    }
+    if ((st = drain()) != PP_OK) return st;
+    return debug_check(s);
+}
+
+}  // namespace
+
 pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channels, pp_direction direction,
                        const pp_policy* policy, const pp_options* options, uint64_t* layer_len) {
     if (s) PP_NOT_CAPTURING(s->ctx, "pp_propagate");
@@ -2751,6 +2890,8 @@ pp_status pp_propagate(pp_sum* s, const pp_channel* channels, uint64_t n_channel
     if ((st = check_channels(s, channels, n_channels)) != PP_OK) return st;
     const RunPlanner plan{channels, n_channels, direction == PP_HEISENBERG, policy, target, min_buckets};
     const KeepProgram keep = lower_policy(policy);
+    if (!plan.finalizes && !class_rotation_runs() && lagged_lengths())
+        return propagate_lagged(s, plan, keep, layer_len);
     uint64_t i = 0, run_end = 0;  // layer i belongs to the planner's run ending at run_end
     std::vector<Prepared> window;
     std::vector<uint64_t> lens;

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -39,8 +40,6 @@ constexpr int kRunDimension = 3;                  // engine/bucketed/rotation_la
 constexpr uint32_t kMaxPairRows = 1024;           // the largest pair a join's redo launch stages
 // Device counters of a sum: [0] rows, [1] longest bucket, [2] join overflow, [3] offsets total, [4] redo pairs.
 constexpr int kStatSlots = 8;
-// The sum's pinned copies of them: one for a synchronous read, two for the counters of layers queued ahead of them.
-constexpr int kPinnedStatSlots = 3 * kStatSlots;
 
 // Defaults of the per-context tuning, each overridable from the environment.
 // The join's grid (PP_JOIN_GROUPS_PER_CU): one workgroup per pair by default (measured fastest on the saturated
@@ -369,6 +368,11 @@ struct pp_sum {
     uint64_t tiles_cap = 0;
     unsigned long long* d_stats = nullptr;
     unsigned long long* h_stats = nullptr;  // pinned
+    // Two pinned, device-visible slots of kStatSlots words for the counters of layers queued ahead of them (lagged
+    // lengths), allocated on first use; the device address of the same memory; the small join's sequence number.
+    unsigned long long* h_lag = nullptr;
+    unsigned long long* d_lag_host = nullptr;
+    unsigned long long lag_seq = 0;
     pp::TopnState* d_topn = nullptr;
     unsigned long long* d_hist = nullptr;
     uint32_t* d_redo = nullptr;  // pairs a join could not stage, for its redo launch
@@ -1086,6 +1090,59 @@ uint64_t join_output_rows(const pp_sum* src) {
     return 2 * rows_held + (uint64_t)kSegAlign * (1ull << src->device_bits);
 }
 
+// LDS hash-table slots of a join stage of `rows` rows: a power of two at least twice the rows.
+uint32_t join_table_bits(uint32_t rows) {
+    uint32_t b = 1;
+    while ((1u << b) < 2u * rows) ++b;
+    return b;
+}
+
+// The largest pair a join from src can meet, a multiple of 64 rows (at most kMaxPairRows).
+uint32_t join_pair_bound(const pp_sum* src, uint32_t delta) {
+    uint32_t lcap = delta ? 2u * src->max_bucket : src->max_bucket;
+    lcap = std::max<uint32_t>(64, (lcap + 63) & ~63u);
+    return std::min(lcap, kMaxPairRows);
+}
+
+// A join's arguments from src's current columns into dst's column set `out`, staging `scap` rows; the counters,
+// redo queue, totals mode and filter are left for the caller.
+template <int W, typename T>
+JoinArgs<W, T> join_args(const pp_sum* src, const pp_sum* dst, int out, const Prepared& p, const KeepProgram& keep,
+                         uint32_t delta, uint32_t scap) {
+    const int in = src->cur;
+    const uint32_t B = 1u << src->device_bits;
+    JoinArgs<W, T> a{};
+    a.x = src->col[in].x;
+    a.z = src->col[in].z;
+    a.c = (const T*)src->col[in].c;
+    a.cap = src->col[in].cap;
+    a.start = src->bk[in].start;
+    a.len = src->bk[in].len;
+    a.ox = dst->col[out].x;
+    a.oz = dst->col[out].z;
+    a.oc = (T*)dst->col[out].c;
+    a.ocap = dst->col[out].cap;
+    a.ostart = dst->bk[out].start;
+    a.olen = dst->bk[out].len;
+    a.delta = delta;
+    a.hb = delta ? 31u - (uint32_t)__builtin_clz(delta) : 0u;
+    a.scap = scap;
+    a.tcap_bits = join_table_bits(scap);
+    a.num_cosets = delta ? B / 2 : B;
+    uint32_t gphase = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+        a.gx[w] = p.gx[w];
+        a.gz[w] = p.gz[w];
+        gphase += (uint32_t)__builtin_popcountll(p.gx[w] & p.gz[w]);
+    }
+    a.gphase = gphase & 3u;
+    a.cos_t = (T)p.cos;
+    a.sin_t = (T)p.sin;
+    a.clifford = p.clifford ? 1 : 0;
+    a.keep = keep;
+    return a;
+}
+
 pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, const KeepProgram& keep,
                       const JoinScratch* fixed = nullptr) {
     pp_context* ctx = src->ctx;
@@ -1109,15 +1166,8 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, cons
     // lcap bounds the largest pair; the first launch stages scap anticommuting rows per pair, and the pairs with
     // more (rare: a generator anticommutes with a small fraction of a random bucket) go to a redo launch that
     // stages lcap.  LDS per workgroup, and so the resident workgroups per CU, follow scap.
-    uint32_t lcap = delta ? 2u * src->max_bucket : src->max_bucket;
-    lcap = std::max<uint32_t>(64, (lcap + 63) & ~63u);
-    if (lcap > kMaxPairRows) lcap = kMaxPairRows;
+    const uint32_t lcap = join_pair_bound(src, delta);
     const uint32_t scap = join_stage_rows(ctx->tuning.stage_rows, lcap);
-    auto table_bits_for = [](uint32_t rows) {
-        uint32_t b = 1;
-        while ((1u << b) < 2u * rows) ++b;
-        return b;
-    };
     const uint32_t cosets = delta ? B / 2 : B;
     // sized for B whatever this channel's cosets, so that no later layer on the same bucketing reallocates it (a
     // captured graph holds the pointer)
@@ -1144,35 +1194,7 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, cons
     hipError_t e = dispatch(src, [&](auto Wc, auto Tv) {
         constexpr int W = decltype(Wc)::value;
         using T = decltype(Tv);
-        JoinArgs<W, T> a{};
-        a.x = src->col[in].x;
-        a.z = src->col[in].z;
-        a.c = (const T*)src->col[in].c;
-        a.cap = src->col[in].cap;
-        a.start = src->bk[in].start;
-        a.len = src->bk[in].len;
-        a.ox = dst->col[out].x;
-        a.oz = dst->col[out].z;
-        a.oc = (T*)dst->col[out].c;
-        a.ocap = dst->col[out].cap;
-        a.ostart = dst->bk[out].start;
-        a.olen = dst->bk[out].len;
-        a.delta = delta;
-        a.hb = delta ? 31u - (uint32_t)__builtin_clz(delta) : 0u;
-        a.scap = scap;
-        a.tcap_bits = table_bits_for(scap);
-        a.num_cosets = cosets;
-        uint32_t gphase = 0;
-        for (uint32_t w = 0; w < W; ++w) {
-            a.gx[w] = p.gx[w];
-            a.gz[w] = p.gz[w];
-            gphase += (uint32_t)__builtin_popcountll(p.gx[w] & p.gz[w]);
-        }
-        a.gphase = gphase & 3u;
-        a.cos_t = (T)p.cos;
-        a.sin_t = (T)p.sin;
-        a.clifford = p.clifford ? 1 : 0;
-        a.keep = keep;
+        JoinArgs<W, T> a = join_args<W, T>(src, dst, out, p, keep, delta, scap);
         a.stats = stats;
         a.redo = redo;
         a.redo_count = stats + 4;
@@ -1184,7 +1206,7 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, cons
         if (le != hipSuccess || scap >= lcap) return le;  // a first launch staging lcap never queues a pair
         a.redo_mode = 1;
         a.scap = lcap;
-        a.tcap_bits = table_bits_for(lcap);
+        a.tcap_bits = join_table_bits(lcap);
         return launch_rotation_join<W, T>(ctx->stream, a, 0, ctx->tuning.join_block);
     });
     PP_HIP(ctx, e);
@@ -2110,8 +2132,7 @@ pp_status pp_sum_upload(pp_context* ctx, uint32_t num_qubits, uint32_t width, pp
     pp_status st;
     if ((st = dev_alloc(ctx, (void**)&s->d_rows, s->rows.size() * 8)) != PP_OK) return bail(st);
     if ((st = dev_alloc(ctx, (void**)&s->d_stats, kStatSlots * sizeof(unsigned long long))) != PP_OK) return bail(st);
-    if (hipHostMalloc((void**)&s->h_stats, kPinnedStatSlots * sizeof(unsigned long long), hipHostMallocDefault) !=
-        hipSuccess)
+    if (hipHostMalloc((void**)&s->h_stats, kStatSlots * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return bail(fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned host allocation"));
     if (hipMemcpy(s->d_rows, s->rows.data(), s->rows.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(ctx, PP_ERR_DRIVER, "hash rows upload"));
@@ -2220,6 +2241,7 @@ pp_status pp_sum_free(pp_sum* s) {
     if (s->d_rows) (void)hipFree(s->d_rows);
     if (s->d_stats) (void)hipFree(s->d_stats);
     if (s->h_stats) (void)hipHostFree(s->h_stats);
+    if (s->h_lag) (void)hipHostFree(s->h_lag);
     if (s->d_topn) (void)hipFree(s->d_topn);
     if (s->d_redo) (void)hipFree(s->d_redo);
     if (s->d_score) (void)hipFree(s->d_score);
@@ -2260,8 +2282,7 @@ pp_status pp_sum_clone(const pp_sum* src_c, pp_sum** out) {
     pp_status st;
     if ((st = dev_alloc(ctx, (void**)&s->d_rows, s->rows.size() * 8)) != PP_OK) return bail(st);
     if ((st = dev_alloc(ctx, (void**)&s->d_stats, kStatSlots * sizeof(unsigned long long))) != PP_OK) return bail(st);
-    if (hipHostMalloc((void**)&s->h_stats, kPinnedStatSlots * sizeof(unsigned long long), hipHostMallocDefault) !=
-        hipSuccess)
+    if (hipHostMalloc((void**)&s->h_stats, kStatSlots * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return bail(fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned host allocation"));
     const Columns& sc = src->col[src->cur];
     const BucketArrays& sb = src->bk[src->cur];
@@ -2764,6 +2785,41 @@ bool lagged_lengths() {
     return !(e && std::strcmp(e, "0") == 0);
 }
 
+// Whether layer p from s can take the one-launch small join: a layout of at most kSmallJoinBuckets buckets, every
+// pair staged whole (no redo launch), and no measurement (which times the usual launches).
+bool small_join_fits(const pp_sum* s, const Prepared& p) {
+    const uint32_t B = 1u << s->device_bits;
+    if (B > kSmallJoinBuckets || s->ctx->profiling) return false;
+    const uint32_t delta = bucket_of_host(s, p.gx, p.gz, s->device_bits);
+    const uint32_t lcap = join_pair_bound(s, delta);
+    return join_stage_rows(s->ctx->tuning.stage_rows, lcap) >= lcap;
+}
+
+// Queue the small join of layer p from s's current columns into column set `out`: the counters arrive in the
+// pinned slot whose device address is `host`, followed by `seq`.  s's counters (stats 0, 1, 2 and 5) are zero.
+pp_status launch_join_small(pp_sum* s, int out, const Prepared& p, const KeepProgram& keep,
+                            unsigned long long* host, unsigned long long seq) {
+    pp_context* ctx = s->ctx;
+    const uint32_t B = 1u << s->device_bits;
+    const uint32_t delta = bucket_of_host(s, p.gx, p.gz, s->device_bits);
+    const uint32_t lcap = join_pair_bound(s, delta);
+    hipError_t e = dispatch(s, [&](auto Wc, auto Tv) {
+        constexpr int W = decltype(Wc)::value;
+        using T = decltype(Tv);
+        SmallJoinArgs<W, T> a{};
+        a.j = join_args<W, T>(s, s, out, p, keep, delta, lcap);
+        a.j.stats = s->d_stats;
+        a.j.count_totals = 1;
+        a.orows = s->bk[out].rows;
+        a.num_buckets = B;
+        a.host = host;
+        a.seq = seq;
+        return launch_rotation_join_small<W, T>(ctx->stream, a);
+    });
+    PP_HIP(ctx, e);
+    return PP_OK;
+}
+
 // run_layers under a policy with no layer pass, its rotation layers queued ahead of their counts.  The planner's
 // bookkeeping (the reference's bucket count at every run start) runs as each length arrives.
 pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram& keep, uint64_t* layer_len) {
@@ -2771,11 +2827,20 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
     pp_status st;
     for (hipEvent_t& e : ctx->lag_ev)
         if (!e) PP_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (!s->h_lag) {
+        if (hipHostMalloc((void**)&s->h_lag, 2 * kStatSlots * sizeof(unsigned long long),
+                          hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+            return fail(ctx, PP_ERR_OUT_OF_MEMORY, "pinned host allocation");
+        std::memset(s->h_lag, 0, 2 * kStatSlots * sizeof(unsigned long long));
+        PP_HIP(ctx, hipHostGetDevicePointer((void**)&s->d_lag_host, s->h_lag, 0));
+    }
     const uint64_t n = plan.n_channels;
     const uint64_t target = ctx->tuning.bucket_target;
     uint64_t i = 0, run_end = 0;  // layer i is the next to queue; the planner's current run ends at run_end
     uint32_t queued = 0, head = 0;  // layers whose counters are on their way; ring slot of the oldest
     uint64_t queued_layer[2] = {};
+    unsigned long long queued_seq[2] = {};  // a small join's sequence number in its slot, or 0 (a copied slot)
+    bool counters_clear = false;  // s's counters are zero (a small join leaves them so)
     uint64_t known_len = s->len, known_mb = s->max_bucket;  // counts after the last layer read
     // the planner's bookkeeping at layer r, once the length before it is known
     auto plan_at = [&](uint64_t r, uint64_t len_before) {
@@ -2787,8 +2852,20 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
     };
     auto read_oldest = [&]() -> pp_status {
         const uint32_t k = head;
-        PP_HIP(ctx, hipEventSynchronize(ctx->lag_ev[k]));
-        const unsigned long long* h = s->h_stats + kStatSlots * (1 + k);
+        volatile const unsigned long long* h = s->h_lag + kStatSlots * k;
+        if (queued_seq[k]) {
+            // the small join's last workgroup writes the slot, then the sequence number: wait for it on the host,
+            // and after a long wait for the launch itself (which reports a failed launch)
+            const auto t0 = std::chrono::steady_clock::now();
+            while (h[4] != queued_seq[k])
+                if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
+                    PP_HIP(ctx, hipEventSynchronize(ctx->lag_ev[k]));
+                    if (h[4] != queued_seq[k]) return fail(ctx, PP_ERR_DRIVER, "small join: counters never arrived");
+                    break;
+                }
+        } else {
+            PP_HIP(ctx, hipEventSynchronize(ctx->lag_ev[k]));
+        }
         if (h[2]) return fail(ctx, PP_ERR_DRIVER, "rotation join: a pair exceeded the largest-pair bound");
         known_len = h[0];
         known_mb = h[1];
@@ -2824,6 +2901,7 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
                 if ((st = refine_once(s)) != PP_OK) return st;
             known_len = s->len;
             known_mb = s->max_bucket;
+            counters_clear = false;  // a rebucketing counts with them
         }
         const uint64_t len_hi = known_len << queued, mb_hi = known_mb << queued;
         lag = lag && len_hi <= kLagMaxRows && 2 * mb_hi <= kMaxPairRows &&
@@ -2831,13 +2909,16 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
         const int out = 1 - s->cur;
         const uint32_t B = 1u << s->device_bits;
         const uint64_t need = 2 * len_hi + (uint64_t)kSegAlign * B;
-        if (lag && (s->col[out].cap < need || !s->col[out].x || s->bk[out].cap < B || !s->bk[out].start)) {
+        // with nothing queued, this column set also takes room for a layer queued behind one more (the doubled
+        // bound), so that the queue does not drain again at the next layer to write here
+        const uint64_t want = queued ? need : 4 * len_hi + (uint64_t)kSegAlign * B;
+        if (lag && (s->col[out].cap < want || !s->col[out].x || s->bk[out].cap < B || !s->bk[out].start)) {
             // the output columns must grow: not under a queued layer, which may still read them
             if (queued) {
                 if ((st = drain()) != PP_OK) return st;
                 continue;
             }
-            if ((st = ensure_columns(s, out, 4 * need)) != PP_OK) return st;  // room for the layers that follow
+            if ((st = ensure_columns(s, out, 2 * want)) != PP_OK) return st;  // room for the layers that follow
             if ((st = ensure_buckets(s, out, B)) != PP_OK) return st;
         }
         if (!lag) {
@@ -2849,16 +2930,27 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
             if ((st = debug_check(s)) != PP_OK) return st;
             known_len = s->len;
             known_mb = s->max_bucket;
+            counters_clear = false;
             i = end;
             continue;
         }
         // queue layer i, sized from the bounds; its counters follow it to the next ring slot
         s->len = len_hi;
         s->max_bucket = (uint32_t)mb_hi;
-        if ((st = launch_join(s, s, out, p, keep)) != PP_OK) return st;
         const uint32_t k = (head + queued) & 1u;
-        PP_HIP(ctx, hipMemcpyAsync(s->h_stats + kStatSlots * (1 + k), s->d_stats,
-                                   kStatSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+        if (small_join_fits(s, p)) {  // one launch: the join lays out its output and publishes its counters
+            if (!counters_clear) PP_HIP(ctx, launch_fill_u64(ctx->stream, s->d_stats, kStatSlots, 0ull));
+            queued_seq[k] = ++s->lag_seq;
+            if ((st = launch_join_small(s, out, p, keep, s->d_lag_host + kStatSlots * k, queued_seq[k])) != PP_OK)
+                return st;
+            counters_clear = true;
+        } else {
+            if ((st = launch_join(s, s, out, p, keep)) != PP_OK) return st;
+            PP_HIP(ctx, hipMemcpyAsync(s->h_lag + kStatSlots * k, s->d_stats, kStatSlots * sizeof(unsigned long long),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+            queued_seq[k] = 0;
+            counters_clear = false;
+        }
         PP_HIP(ctx, hipEventRecord(ctx->lag_ev[k], ctx->stream));
         queued_layer[k] = i;
         ++queued;

@@ -51,6 +51,21 @@ struct JoinArgs {
     const float* filter_score;    // its ranking score per input position when it ranked by score, else null (|c|)
 };
 
+// A small layer's join in one launch (k_rotation_join_small, a layout of at most kSmallJoinBuckets buckets, one
+// workgroup per pair): every workgroup lays out its own pair's output segments from the input lengths, joins the
+// pair, and the last workgroup to finish publishes the layer's counters to a pinned host slot and clears them, and
+// its finish counter stats[5], for the next launch.  j.stats[0, 1, 2, 5] are zero before the launch.
+constexpr uint32_t kSmallJoinBuckets = 2048;
+template <int W, typename T>
+struct SmallJoinArgs {
+    JoinArgs<W, T> j;          // j.ostart and j.olen are written; j.count_totals set; no redo queue
+    uint32_t* orows;           // each output segment's room in rows
+    uint32_t num_buckets;
+    unsigned long long* host;  // pinned, device-visible: [0] rows, [1] longest bucket, [2] overflow, [3] layout end,
+                               // [4] seq once [0 .. 3] are written
+    unsigned long long seq;
+};
+
 // The rotation layer in place (k_rotation_inplace): the sum's own columns and bucket arrays are read and written.
 template <int W, typename T>
 struct InplaceArgs {
@@ -277,6 +292,8 @@ constexpr uint32_t kJoinOnePairPerGroup = 0xFFFFFFFFu;
 // block: 256 or 512 threads per workgroup (a chunk of that many rows per pass).
 template <int W, typename T>
 hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t groups_per_cu, uint32_t block);
+template <int W, typename T>
+hipError_t launch_rotation_join_small(hipStream_t st, const SmallJoinArgs<W, T>& a);
 // One layer for each of num_items sums (device arrays: items, pair_start = exclusive prefix of their pair counts,
 // total_pairs in all): their output offsets, the joins (each item staging its largest pair, scap = lcap), and
 // their counters, also gathered into out[4 i + 0..2] = rows, longest bucket, overflow flag.

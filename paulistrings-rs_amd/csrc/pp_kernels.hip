@@ -390,9 +390,82 @@ __device__ __forceinline__ void join_staged_pass(const JoinArgs<W, T>& a, const 
 // The join of work items t, t + G, t + G + G, ... below count (see k_rotation_join); lds and scan_scratch are the
 // workgroup's.  FILTER: the input may carry a pending TopN selection (a compile-time switch: the arguments it reads
 // stay out of the scalar registers of the common launch, which has none).
-template <int W, typename T, bool REDO, int BLOCK, bool FILTER>
+// The output segment sizes of a small layer (at most kSmallJoinBuckets buckets) below a pair's two buckets and in
+// all, from the input lengths: each thread's share of the buckets is loaded first (small_lens_load) and summed
+// later (small_lens_sum), so that the loads overlap the pair's own first loads.
+constexpr uint32_t kSmallLensPerThread = kSmallJoinBuckets / kBlock;
+struct SmallLens {
+    uint32_t n[kSmallLensPerThread];  // rows of bucket i and its partner i ^ delta
+};
+
+template <int W, typename T>
+__device__ __forceinline__ SmallLens small_lens_load(const JoinArgs<W, T>& a, uint32_t num_buckets) {
+    SmallLens l;
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallLensPerThread; ++k) {
+        const uint32_t i = threadIdx.x + k * kBlock;
+        l.n[k] = 0;
+        if (i < num_buckets) l.n[k] = a.delta ? a.len[i] + a.len[i ^ a.delta] : 2u * a.len[i];
+    }
+    return l;
+}
+
+// Sizes below the pair's b0, below its b1 and in all (a workgroup reduction through part[3][kWaves]) into m.o0,
+// m.o1 and all; thread 0 also writes the starts and rooms of b0 and b1.
+template <int W, typename T>
+__device__ __forceinline__ void small_lens_sum(const JoinArgs<W, T>& a, const SmallLens& l, PairMeta& m,
+                                               uint32_t* orows, uint64_t* part, uint64_t& all) {
+    const uint32_t b0 = m.b0, b1 = m.b1;
+    uint64_t below0 = 0, below1 = 0, tot = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallLensPerThread; ++k) {
+        const uint32_t i = threadIdx.x + k * kBlock;
+        const uint64_t sz = ((uint64_t)l.n[k] + (kSegAlign - 1)) & ~(uint64_t)(kSegAlign - 1);
+        tot += sz;
+        below0 += i < b0 ? sz : 0ull;
+        below1 += i < b1 ? sz : 0ull;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        below0 += __shfl_down(below0, d, 64);
+        below1 += __shfl_down(below1, d, 64);
+        tot += __shfl_down(tot, d, 64);
+    }
+    if (lane_id() == 0) {
+        part[wave_id()] = below0;
+        part[kWaves + wave_id()] = below1;
+        part[2 * kWaves + wave_id()] = tot;
+    }
+    __syncthreads();
+    uint64_t o0 = 0, o1 = 0;
+    all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        o0 += part[w];
+        o1 += part[kWaves + w];
+        all += part[2 * kWaves + w];
+    }
+    m.o0 = o0;
+    m.o1 = o1;
+    if (threadIdx.x == 0) {
+        const uint64_t n = a.delta ? (uint64_t)m.n0 + m.n1 : 2ull * m.n0;  // the pair's rows (n1 = 0 alone)
+        const uint32_t room = (uint32_t)((n + (kSegAlign - 1)) & ~(uint64_t)(kSegAlign - 1));
+        const_cast<uint64_t*>(a.ostart)[b0] = o0;
+        orows[b0] = room;
+        if (a.delta) {
+            const_cast<uint64_t*>(a.ostart)[b1] = o1;
+            orows[b1] = room;
+        }
+    }
+}
+
+// SMALL: one pair per workgroup, which lays out its own output segments (small_lens_*: small_buckets buckets,
+// small_rows receiving the rooms, small_part the reduction's LDS, *small_all the layout's end).
+template <int W, typename T, bool REDO, int BLOCK, bool FILTER, bool SMALL = false>
 __device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, uint32_t G, uint32_t count,
-                                           unsigned char* lds, uint32_t* scan_scratch) {
+                                           unsigned char* lds, uint32_t* scan_scratch, uint32_t small_buckets = 0,
+                                           uint32_t* small_rows = nullptr, uint64_t* small_part = nullptr,
+                                           uint64_t* small_all = nullptr) {
     const uint32_t scap = a.scap;  // staged rows this launch holds
     uint64_t* const Lx = reinterpret_cast<uint64_t*>(lds);   // [W][scap]
     uint64_t* const Lz = Lx + (size_t)W * scap;              // [W][scap]
@@ -422,10 +495,14 @@ __device__ __forceinline__ void join_items(const JoinArgs<W, T>& a, uint32_t t, 
     }
 
     uint32_t p = pair_at(t);
-    PairMeta m = read_meta(a, p, load_meta(a, p));
+    const MetaWords w0 = load_meta(a, p);
+    SmallLens sl;
+    if constexpr (SMALL) sl = small_lens_load(a, small_buckets);
+    PairMeta m = read_meta(a, p, w0);
     ChunkRow<W, T> cb;
     load_chunk_row(a, m, threadIdx.x, cb);
     MetaWords mw = load_meta(a, pair_at(min(t + G, last)));
+    if constexpr (SMALL) small_lens_sum(a, sl, m, small_rows, small_part, *small_all);
     // Land the first chunk here, so that inside the loop the current chunk is always one copied from the
     // previous iteration's prefetch (which the copy waited for) and its first use waits on nothing younger.
     __builtin_amdgcn_s_waitcnt(0);
@@ -559,6 +636,43 @@ __global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a)
     __shared__ uint32_t scan_scratch[4 * (BLOCK / 64)];
     const uint32_t count = REDO ? (uint32_t)*a.redo_count : a.num_cosets;
     join_items<W, T, REDO, BLOCK, FILTER>(a, blockIdx.x, gridDim.x, count, lds, scan_scratch);
+}
+
+// A small layer's join in one launch (see SmallJoinArgs).  The workgroup of pair p sums the output segment sizes
+// (each pair's rows rounded up to kSegAlign, as the offset scan lays them out) below each of its two buckets and in
+// all, over every bucket of the layout (at most kSmallJoinBuckets, read by every workgroup while its pair's first
+// rows load), writes its buckets' starts and rooms, and joins the pair.  The last workgroup to finish (a finish counter in stats[5])
+// copies the counters to the host slot, then its sequence number, and clears them for the next launch.
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_rotation_join_small(const SmallJoinArgs<W, T> a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t scan_scratch[4 * kWaves];
+    __shared__ uint64_t part[3][kWaves];
+    const JoinArgs<W, T>& j = a.j;
+    uint64_t all = 0;
+    join_items<W, T, false, kBlock, false, true>(j, blockIdx.x, gridDim.x, j.num_cosets, lds, scan_scratch,
+                                                 a.num_buckets, a.orows, &part[0][0], &all);
+    // Only atomics order the finish: the counters are device-scope atomics, done (waited for) before this
+    // workgroup's finish is counted, and the last workgroup reads them back with atomics.  No fence: a device-scope
+    // fence in every workgroup writes back its L2.  The rows and lengths reach the next kernel at the kernel
+    // boundary; the host reads only the slot.
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_s_waitcnt(0);
+        if (atomicAdd(&j.stats[5], 1ull) == gridDim.x - 1ull) {
+            const unsigned long long rows = atomicExch(&j.stats[0], 0ull);
+            const unsigned long long longest = atomicExch(&j.stats[1], 0ull);
+            const unsigned long long flags = atomicExch(&j.stats[2], 0ull);
+            atomicExch(&j.stats[5], 0ull);
+            j.stats[3] = all;
+            volatile unsigned long long* h = a.host;  // fine-grained host memory: the stores bypass the L2
+            h[0] = rows;
+            h[1] = longest;
+            h[2] = flags;
+            h[3] = all;
+            __builtin_amdgcn_s_waitcnt(0);  // the slot's words land before its sequence number
+            h[4] = a.seq;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1456,6 +1570,15 @@ hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_
     return hipGetLastError();
 }
 
+template <int W, typename T>
+hipError_t launch_rotation_join_small(hipStream_t st, const SmallJoinArgs<W, T>& a) {
+    if (a.j.num_cosets == 0 || a.num_buckets > kSmallJoinBuckets || a.j.redo_mode || a.j.filter || !a.j.count_totals)
+        return hipErrorInvalidValue;
+    const size_t lds = join_lds_bytes<W, T>(a.j.scap, a.j.tcap_bits);
+    hipLaunchKernelGGL((k_rotation_join_small<W, T>), dim3(a.j.num_cosets), dim3(kBlock), lds, st, a);
+    return hipGetLastError();
+}
+
 // One workgroup per pair; the redo launch, over the queued pairs, is a grid of as many as stay resident.
 template <int W, typename T>
 hipError_t launch_rotation_inplace(hipStream_t st, const InplaceArgs<W, T>& a) {
@@ -1607,6 +1730,7 @@ hipError_t launch_check_layout(hipStream_t st, const CheckArgs<W, T>& a) {
 #define PP_INSTANTIATE(W, T)                                                                           \
     template size_t join_lds_bytes<W, T>(uint32_t, uint32_t);                                          \
     template hipError_t launch_rotation_join<W, T>(hipStream_t, const JoinArgs<W, T>&, uint32_t, uint32_t); \
+    template hipError_t launch_rotation_join_small<W, T>(hipStream_t, const SmallJoinArgs<W, T>&);      \
     template hipError_t launch_join_batch<W, T>(hipStream_t, const JoinArgs<W, T>*, const uint32_t*, uint32_t,     \
                                                 uint32_t, uint32_t, uint32_t, unsigned long long*);             \
     template hipError_t launch_rescale<W, T>(hipStream_t, const RescaleArgs<W, T>&, uint32_t);          \

@@ -53,6 +53,8 @@ def run(monkeypatch, lagged, num_qubits, x, z, c, circ, direction, policy):
     monkeypatch.setenv("PP_LAGGED_LENGTHS", "1" if lagged else "0")
     s = PauliSum(num_qubits, x, z, c).to_device(0)
     stats = s.propagate_with_stats(circ, policy=policy, direction=direction)
+    bad = {k: v for k, v in s.check_layout().items() if v and k != "unscanned"}
+    assert not bad, bad  # segments, rooms and lengths the queued layers laid out on the device
     bits = s.bucket_bits()
     h = s.to_host()
     s.free()
@@ -151,3 +153,38 @@ def test_empty_and_vanishing_sums(monkeypatch):
     assert lag[3][-1] == 0
     none = run(monkeypatch, True, 10, x, z, c, Circuit(10), "forward", pol)
     assert none[3] == [] and len(none[2]) == 1
+
+
+def test_in_place_layer_after_a_lagged_propagation(monkeypatch):
+    """The layout the one-launch small joins leave (segment starts and rooms, the end of the last segment) carries
+    the in-place layer that follows, which writes new rows into the rooms and after that end."""
+    announce("lagged lengths: an in-place layer after the queue")
+    rng = np.random.default_rng(21)
+    x, z, c = random_sum(rng, 500, 64, 0.1)
+    circ = Circuit(64)
+    for q in range(10):
+        circ.rx(0.4 + 0.05 * q, q)
+    more = Circuit(64).rzz(0.9, 3, 40).rx(0.3, 11)
+    pol = truncation.coeff(1e-8)
+    monkeypatch.setenv("PP_LAGGED_LENGTHS", "1")
+    s = PauliSum(64, x, z, c).to_device(0)
+    s.propagate(circ, policy=pol, direction="forward")
+    monkeypatch.setenv("PP_IN_PLACE_MIN_ROWS", "0")
+    for ch in more.channels():
+        s.apply_rotation(ch, policy=pol)
+    bad = {k: v for k, v in s.check_layout().items() if v and k != "unscanned"}
+    assert not bad, bad
+    h = s.to_host()
+    s.free()
+    whole = Circuit(64)
+    for q in range(10):
+        whole.rx(0.4 + 0.05 * q, q)
+    whole.rzz(0.9, 3, 40).rx(0.3, 11)
+    o = oracle(64, x, z, c, whole, "forward", eps=1e-8)
+
+    def by_key(kx, kz, kc):  # the same terms whatever the bucket count behind the canonical order
+        order = np.lexsort((kz[:, 0], kx[:, 0]))
+        return kx[order], kz[order], kc[order]
+
+    got, want = by_key(h.x_array(), h.z_array(), h.coefficients_array()), by_key(o[0], o[1], o[2])
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))

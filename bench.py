@@ -433,6 +433,61 @@ def _oracle_threads() -> int:
     return max(1, min(16, os.cpu_count() or 1))
 
 
+def transfer_line(steps: int, log2n: int) -> dict:
+    """pp_sum_upload and pp_sum_download of 2^log2n distinct random keys on 127 qubits (W = 2, f64) through the
+    context's pinned staging, after one untimed round trip (which also allocates the staging).  The host object is
+    built from the arrays directly: the keys are distinct draws, so the Python constructor's canonicalisation (a
+    lexsort, outside the engine) is left out."""
+    import numpy as np
+
+    import paulistrings_amd as pa
+
+    rng = np.random.default_rng(SEED)
+    n = 1 << log2n
+    h = pa.PauliSum.__new__(pa.PauliSum)
+    h.num_qubits, h.width, h.hash_seed = 127, 2, pa.DEFAULT_HASH_SEED
+    h._x = rng.integers(0, 1 << 62, size=(n, 2), dtype=np.uint64)
+    h._z = rng.integers(0, 1 << 62, size=(n, 2), dtype=np.uint64)
+    h._c = rng.standard_normal(n)
+    up = down = 0.0
+    for rep in range(steps + 1):
+        t0 = time.perf_counter()
+        g = h.to_device(0)
+        g.ctx.sync()
+        t1 = time.perf_counter()
+        back = g.to_host()
+        t2 = time.perf_counter()
+        g.free()
+        if len(back) != n:
+            raise RuntimeError(f"round trip lost terms: {len(back)} of {n}")
+        if rep:
+            up += t1 - t0
+            down += t2 - t1
+    up /= max(1, steps)
+    down /= max(1, steps)
+    nbytes = n * (2 * 2 * 8 + 8)
+    log(f"transfer: upload {up:.3f} s, download {down:.3f} s for {n} terms ({nbytes / 1e6:.0f} MB)")
+    return {
+        "metric": "host-device round trip of a sum (canonical upload and download)",
+        "value": n / (up + down),
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": (up + down) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2^{log2n} distinct random keys on 127 qubits, f64, upload then download",
+                   "parallelism": "single GPU"},
+        "roofline": None,
+        "cpu_baseline": None,
+        "detail": {"upload_s": up, "download_s": down, "bytes": nbytes},
+    }
+
+
 def tfim6x6_line(steps: int, warmup: int) -> dict:
     """The reference's CPU-runnable case: the average-X observable on the 6 x 6 periodic TFIM (J = 1, h = 3, dt = 0.1,
     10 Trotter steps of 72 RZZ + 36 RX), Heisenberg, CoefficientThreshold(1e-4) and weight <= 8, read out in |+>^36.
@@ -681,13 +736,14 @@ def main() -> int:
                     help="only the in-place line (no out-of-place comparison, no float32 line, no STREAM probe): "
                          "for profiler runs")
     ap.add_argument("--workload", choices=["saturated", "learned-score", "batched", "tfim6x6", "kicked-ising",
-                                           "rotation-run", "general-table"],
+                                           "rotation-run", "general-table", "transfer"],
                     default="saturated",
                     help="saturated: the headline line (default); learned-score: the learned pass's MLP scoring on "
                          "2^22 terms; batched: 64 independent propagations stepped together; tfim6x6: the 6x6 TFIM "
                          "against the oracle; kicked-ising: the 127-qubit heavy-hex kicked Ising with TopN 2^24; "
                          "rotation-run / general-table: the class pass on the saturated input (eight RX as one run "
-                         "against one by one; one dense two-qubit unitary)")
+                         "against one by one; one dense two-qubit unitary); transfer: upload and download of 2^24 "
+                         "terms through the pinned staging")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference for the HBM "
                          "traffic counters) and exit")
@@ -722,6 +778,9 @@ def main() -> int:
         return 0
     if args.workload == "kicked-ising":
         print(json.dumps(kicked_ising_line(args.cpu_seconds)), flush=True)
+        return 0
+    if args.workload == "transfer":
+        print(json.dumps(transfer_line(args.steps, args.log2n)), flush=True)
         return 0
     if args.workload in ("rotation-run", "general-table"):
         print(json.dumps(class_pass_line(args.workload, args.steps, args.warmup, args.log2n)), flush=True)

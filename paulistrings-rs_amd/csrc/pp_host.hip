@@ -1838,6 +1838,13 @@ pp_status refresh(pp_sum* s) {
 
 pp_status apply_one(pp_sum* s, const Prepared& p, const KeepProgram& keep) { return device_layer(s, p, keep); }
 
+pp_status batch_topn(pp_batch* b, const std::vector<uint32_t>& which, const std::vector<uint64_t>& keep);  // below
+// Whether pp_batch_propagate selects the TopN of every instance that finishes a run in the same step together
+// (default) or one instance after another (PP_BATCH_TOPN=0, for comparison: the same kept terms).
+bool batch_topn_enabled() {
+    const char* e = std::getenv("PP_BATCH_TOPN");
+    return !(e && std::strcmp(e, "0") == 0);
+}
 pp_status topn_device(pp_sum* s, uint64_t n, const float* score = nullptr,
                       bool defer = false);  // pp_topn_impl.inc
 pp_status approx_topn_device(pp_sum* s, uint64_t n);  // pp_topn_impl.inc
@@ -3216,6 +3223,15 @@ struct pp_batch {
     uint32_t* h_pair_start = nullptr;
     unsigned long long* d_out = nullptr;  // 4 counters per item
     unsigned long long* h_out = nullptr;
+    // The batched TopN selection (batch_topn): its item arrays (pinned staging and device copy, one block each),
+    // and its boundary-bin arrays (grow-only; the host copy of the gathered part).
+    void* d_topn_items = nullptr;
+    void* h_topn_items = nullptr;
+    size_t topn_items_bytes = 0;
+    unsigned char* d_ties = nullptr;
+    size_t ties_cap = 0;  // terms
+    std::vector<unsigned char> h_ties;
+    std::vector<uint64_t> h_sorted;  // the host side of the sorted positions' copy (alive until the next sync)
 };
 
 pp_status pp_batch_create(pp_context* ctx, pp_sum* const* sums, uint32_t count, pp_batch** out) {
@@ -3267,6 +3283,9 @@ pp_status pp_batch_free(pp_batch* b) {
     if (b->h_items) (void)hipHostFree(b->h_items);
     if (b->h_pair_start) (void)hipHostFree(b->h_pair_start);
     if (b->h_out) (void)hipHostFree(b->h_out);
+    if (b->d_topn_items) (void)hipFree(b->d_topn_items);
+    if (b->h_topn_items) (void)hipHostFree(b->h_topn_items);
+    if (b->d_ties) (void)hipFree(b->d_ties);
     delete b;
     return PP_OK;
 }
@@ -3430,10 +3449,24 @@ pp_status pp_batch_propagate(pp_batch* b, const pp_channel* channels, const uint
                        : (b->kind == PP_COEFF_F64 ? batch_step<2, double>(b, idx, chans, preps, ks)
                                                  : batch_step<2, float>(b, idx, chans, preps, ks));
         if (st != PP_OK) return st;
-        for (uint32_t i : idx)
-            if (++next[i] == run_end[i] &&
-                (st = plans[i].finish_run(b->sums[i], next[i] < plans[i].n_channels)) != PP_OK)
+        // the layer pass of every instance whose run ended: deferred exact TopN selections of sums above their cap
+        // together, anything else on its own
+        std::vector<uint32_t> together;
+        std::vector<uint64_t> keep;
+        for (uint32_t i : idx) {
+            if (++next[i] != run_end[i]) continue;
+            const RunPlanner& plan = plans[i];
+            pp_sum* s = b->sums[i];
+            const bool defer = next[i] < plan.n_channels;
+            if (batch_topn_enabled() && defer && plan.finalizes && plan.policy->has_topn == 1 &&
+                plan.policy->topn > 0 && s->len > plan.policy->topn && !s->pending_topn) {
+                together.push_back(i);
+                keep.push_back(plan.policy->topn);
+            } else if ((st = plan.finish_run(s, defer)) != PP_OK) {
                 return st;
+            }
+        }
+        if (!together.empty() && (st = batch_topn(b, together, keep)) != PP_OK) return st;
     }
     return PP_OK;
 }

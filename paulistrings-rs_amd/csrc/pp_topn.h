@@ -116,6 +116,32 @@ struct TopnRetainArgs {
     unsigned long long* stats;
 };
 
+// Many selections at once (a batch of sums, each with its own state and histogram): an item per sum.
+struct TopnSelectItem {
+    TopnState* state;
+    unsigned long long* hist;  // kHistWords
+    uint64_t n;                // terms to keep
+    TopnState* out;            // the state's copy after the last pass
+};
+// The boundary flags of one selection: its boundary bin's positions in rank order, the first `kept` of them kept.
+struct TopnFlagItem {
+    const uint64_t* pos;
+    uint64_t ties;
+    uint64_t kept;
+    uint8_t* flags;
+};
+hipError_t launch_topn_init_batch(hipStream_t st, const TopnSelectItem* items, uint32_t num_items);
+// block_start: exclusive prefix of the items' histogram workgroups (num_items + 1 entries, blocks in all).
+template <typename T>
+hipError_t launch_topn_hist_batch(hipStream_t st, const TopnHistArgs<T>* items, const uint32_t* block_start,
+                                  uint32_t num_items, uint32_t blocks, int pass);
+hipError_t launch_topn_select_batch(hipStream_t st, const TopnSelectItem* items, uint32_t num_items, int pass,
+                                    bool last);
+template <int W, typename T>
+hipError_t launch_topn_gather_batch(hipStream_t st, const TopnTieArgs<W, T>* items, const uint32_t* block_start,
+                                    uint32_t num_items, uint32_t blocks);
+hipError_t launch_topn_flags_batch(hipStream_t st, const TopnFlagItem* items, uint32_t num_items);
+
 template <typename T>
 hipError_t launch_topn_hist(hipStream_t st, const TopnHistArgs<T>& a, uint32_t blocks);
 hipError_t launch_topn_select(hipStream_t st, TopnState* state, unsigned long long* hist, int pass, uint64_t n);

@@ -154,14 +154,15 @@ __device__ __forceinline__ void list_positions(const RowList& l, uint32_t total,
     }
 }
 
+// One histogram workgroup's share (workgroup `block` of the selection's grid).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
+__device__ __forceinline__ void topn_hist_body(const TopnHistArgs<T>& a, uint32_t block) {
     __shared__ uint32_t h[kTopnBins];
     __shared__ RowList list;
     const TopnState st = *a.state;
     if (st.done > (unsigned long long)a.pass) return;  // decided by a seeded first pass: the whole workgroup
     for (int i = threadIdx.x; i < kTopnBins; i += kBlock) h[i] = 0;
-    const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
+    const uint32_t b_lo = block * a.buckets_per_block;
     const uint32_t nb = min(a.num_buckets, b_lo + a.buckets_per_block) - b_lo;  // <= kHistBucketsMax
     const uint32_t t = threadIdx.x;
     const uint32_t total = list_rows(a.start, a.len, b_lo, nb, list);
@@ -188,9 +189,36 @@ __global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
         }
     }
     __syncthreads();
-    unsigned long long* const copy = a.hist + (blockIdx.x % kHistCopies) * kTopnBins;
+    unsigned long long* const copy = a.hist + (block % kHistCopies) * kTopnBins;
     for (int i = threadIdx.x; i < kTopnBins; i += kBlock)
         if (h[i]) atomicAdd(&copy[i], (unsigned long long)h[i]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_topn_hist(const TopnHistArgs<T> a) {
+    topn_hist_body<T>(a, blockIdx.x);
+}
+
+// The item of a batched launch whose range [block_start[i], block_start[i + 1]) of workgroups holds blockIdx.x.
+__device__ __forceinline__ uint32_t batch_item(const uint32_t* block_start, uint32_t num_items) {
+    uint32_t lo = 0, hi = num_items;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (block_start[mid] <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Pass `pass` of many selections in one launch (items: each sum's histogram arguments).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_topn_hist_batch(const TopnHistArgs<T>* __restrict__ items,
+                                                             const uint32_t* __restrict__ block_start,
+                                                             uint32_t num_items, int32_t pass) {
+    const uint32_t i = batch_item(block_start, num_items);
+    TopnHistArgs<T> a = items[i];
+    a.pass = pass;
+    topn_hist_body<T>(a, blockIdx.x - block_start[i]);
 }
 
 // One workgroup after each histogram pass (a kernel boundary, not a device-scope fence per histogram workgroup: on
@@ -199,6 +227,42 @@ __global__ void __launch_bounds__(kBlock) k_topn_select(TopnState* s, unsigned l
     if (s->done > (unsigned long long)pass) return;  // uniform: one load, the same for every thread
     select_digit(s, hist, pass, n);
     if (threadIdx.x == 0) s->done = (unsigned long long)pass + 1ull;
+}
+
+// Many selections' states: one workgroup each clears its histogram's first region and starts the state (unseeded).
+__global__ void __launch_bounds__(kBlock) k_topn_init_batch(const TopnSelectItem* __restrict__ items) {
+    const TopnSelectItem it = items[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < kHistWords; i += kBlock) it.hist[i] = 0ull;
+    if (threadIdx.x == 0) {
+        TopnState* s = it.state;
+        s->prefix = 0;
+        s->mask = 0;
+        s->k = it.n;
+        s->above = 0;
+        s->finite = 0;
+        s->ties = 0;
+        s->keep_all_finite = 0;
+        s->gathered = 0;
+        s->done = 0;
+        s->seed = 0;
+        s->seeded = 0;
+    }
+}
+
+// The digit select of pass `pass` for many selections, one workgroup each; with `last`, each state is also copied
+// to its item's slot of the batch's contiguous state array (one read-back for all of them).
+__global__ void __launch_bounds__(kBlock) k_topn_select_batch(const TopnSelectItem* __restrict__ items, int pass,
+                                                              int last) {
+    const TopnSelectItem it = items[blockIdx.x];
+    TopnState* s = it.state;
+    if (s->done <= (unsigned long long)pass) {  // uniform: one load, the same for every thread
+        select_digit(s, it.hist, pass, it.n);
+        if (threadIdx.x == 0) s->done = (unsigned long long)pass + 1ull;
+    }
+    if (last) {
+        __syncthreads();  // the holder's updates of the state
+        if (threadIdx.x == 0) *it.out = *s;
+    }
 }
 
 // The seeded first pass's histograms: region 0 of hist counts the first digit of every term, region 1 the second
@@ -268,10 +332,10 @@ __global__ void __launch_bounds__(kBlock) k_topn_select_seeded(TopnState* s, uns
 // The terms of the boundary bin: rank key equal to the prefix on the decided bits (a NaN never matches a finite
 // prefix: its exponent bits are all ones).  Buckets are walked as in k_topn_hist.
 template <int W, typename T>
-__global__ void __launch_bounds__(kBlock) k_topn_gather_ties(const TopnTieArgs<W, T> a) {
+__device__ __forceinline__ void topn_gather_body(const TopnTieArgs<W, T>& a, uint32_t block) {
     __shared__ RowList list;
     const uint64_t t = a.state->prefix, mask = a.state->mask;
-    const uint32_t b_lo = blockIdx.x * a.buckets_per_block;
+    const uint32_t b_lo = block * a.buckets_per_block;
     const uint32_t nb = min(a.num_buckets, b_lo + a.buckets_per_block) - b_lo;  // <= kHistBucketsMax
     const uint32_t total = list_rows(a.start, a.len, b_lo, nb, list);
     uint32_t j = 0;
@@ -299,6 +363,21 @@ __global__ void __launch_bounds__(kBlock) k_topn_gather_ties(const TopnTieArgs<W
             a.tie_rank[slot] = u[q];
         }
     }
+}
+
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_topn_gather_ties(const TopnTieArgs<W, T> a) {
+    topn_gather_body<W, T>(a, blockIdx.x);
+}
+
+// The boundary bins of many selections in one launch (each item's ties into its own range of the batch's arrays).
+template <int W, typename T>
+__global__ void __launch_bounds__(kBlock) k_topn_gather_batch(const TopnTieArgs<W, T>* __restrict__ items,
+                                                              const uint32_t* __restrict__ block_start,
+                                                              uint32_t num_items) {
+    const uint32_t i = batch_item(block_start, num_items);
+    const TopnTieArgs<W, T> a = items[i];
+    topn_gather_body<W, T>(a, blockIdx.x - block_start[i]);
 }
 
 // Whether the term at column position i (coefficient c) survives the selection: above the boundary bin on the
@@ -351,6 +430,14 @@ __global__ void __launch_bounds__(kBlock) k_topn_retain(const TopnRetainArgs<W, 
 __global__ void k_mark_positions(const uint64_t* pos, uint64_t n, uint8_t* flags) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) flags[pos[i]] = 1;
+}
+
+// The boundary flags of many selections, one workgroup each: every position of the item's boundary bin, in rank
+// order, is flagged kept (its first `kept`) or dropped.  Only positions in the boundary bin are ever read, so no
+// other flag needs a value.
+__global__ void __launch_bounds__(kBlock) k_topn_flags_batch(const TopnFlagItem* __restrict__ items) {
+    const TopnFlagItem it = items[blockIdx.x];
+    for (uint64_t j = threadIdx.x; j < it.ties; j += kBlock) it.flags[it.pos[j]] = j < it.kept ? 1 : 0;
 }
 
 // ---- ApproxTopN ---------------------------------------------------------------------------------------------
@@ -483,6 +570,43 @@ hipError_t launch_octave_retain(hipStream_t st, const OctaveCutArgs<W, T>& a, ui
     return hipGetLastError();
 }
 
+hipError_t launch_topn_init_batch(hipStream_t st, const TopnSelectItem* items, uint32_t num_items) {
+    if (num_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_topn_init_batch, dim3(num_items), dim3(kBlock), 0, st, items);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_topn_hist_batch(hipStream_t st, const TopnHistArgs<T>* items, const uint32_t* block_start,
+                                  uint32_t num_items, uint32_t blocks, int pass) {
+    if (num_items == 0 || blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_topn_hist_batch<T>), dim3(blocks), dim3(kBlock), 0, st, items, block_start, num_items,
+                       (int32_t)pass);
+    return hipGetLastError();
+}
+
+hipError_t launch_topn_select_batch(hipStream_t st, const TopnSelectItem* items, uint32_t num_items, int pass,
+                                    bool last) {
+    if (num_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_topn_select_batch, dim3(num_items), dim3(kBlock), 0, st, items, pass, last ? 1 : 0);
+    return hipGetLastError();
+}
+
+template <int W, typename T>
+hipError_t launch_topn_gather_batch(hipStream_t st, const TopnTieArgs<W, T>* items, const uint32_t* block_start,
+                                    uint32_t num_items, uint32_t blocks) {
+    if (num_items == 0 || blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_topn_gather_batch<W, T>), dim3(blocks), dim3(kBlock), 0, st, items, block_start,
+                       num_items);
+    return hipGetLastError();
+}
+
+hipError_t launch_topn_flags_batch(hipStream_t st, const TopnFlagItem* items, uint32_t num_items) {
+    if (num_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_topn_flags_batch, dim3(num_items), dim3(kBlock), 0, st, items);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_topn_hist(hipStream_t st, const TopnHistArgs<T>& a, uint32_t blocks) {
     hipLaunchKernelGGL((k_topn_hist<T>), dim3(blocks), dim3(kBlock), 0, st, a);
@@ -532,6 +656,10 @@ hipError_t launch_mark_positions(hipStream_t st, const uint64_t* pos, uint64_t n
     return hipGetLastError();
 }
 
+template hipError_t launch_topn_hist_batch<double>(hipStream_t, const TopnHistArgs<double>*, const uint32_t*, uint32_t,
+                                                   uint32_t, int);
+template hipError_t launch_topn_hist_batch<float>(hipStream_t, const TopnHistArgs<float>*, const uint32_t*, uint32_t,
+                                                  uint32_t, int);
 template hipError_t launch_topn_hist<double>(hipStream_t, const TopnHistArgs<double>&, uint32_t);
 template hipError_t launch_topn_hist<float>(hipStream_t, const TopnHistArgs<float>&, uint32_t);
 template hipError_t launch_topn_hist_seeded<double>(hipStream_t, const TopnHistArgs<double>&, uint32_t);
@@ -540,6 +668,8 @@ template hipError_t launch_octave_hist<double>(hipStream_t, const OctaveHistArgs
 template hipError_t launch_octave_hist<float>(hipStream_t, const OctaveHistArgs<float>&, uint32_t);
 #define PP_TOPN_INSTANTIATE(W, T)                                                                       \
     template hipError_t launch_topn_gather_ties<W, T>(hipStream_t, const TopnTieArgs<W, T>&, uint32_t); \
+    template hipError_t launch_topn_gather_batch<W, T>(hipStream_t, const TopnTieArgs<W, T>*, const uint32_t*, \
+                                                       uint32_t, uint32_t);                               \
     template hipError_t launch_topn_retain<W, T>(hipStream_t, const TopnRetainArgs<W, T>&, uint32_t);   \
     template hipError_t launch_octave_counts<W, T>(hipStream_t, const OctaveCutArgs<W, T>&, uint32_t);  \
     template hipError_t launch_octave_gather<W, T>(hipStream_t, const OctaveCutArgs<W, T>&, uint32_t);  \

@@ -110,3 +110,32 @@ def test_batched_workload_at_its_own_parameters():
     pol_o = [po.coeff_policy(eps=eps, topn=cap) for (_n, _c, _o, eps) in inst]
     pol_p = [truncation.coeff(eps) & truncation.topn(cap) for (_n, _c, _o, eps) in inst]
     check_batch(inst, pol_o, pol_p)
+
+
+def test_batched_topn_selections_equal_one_by_one(monkeypatch):
+    """The TopN selections of the instances that finish a run in the same step run together (PP_BATCH_TOPN, the
+    default): one launch per radix pass for all of them, one gather of their boundary bins.  They keep the terms the
+    one-by-one selections keep and the oracle keeps, including an instance whose boundary bin holds more than 2^16
+    exact ties (which takes the one-by-one path) and one whose boundary bin is kept whole."""
+    announce("batched TopN selections vs one by one")
+    rng = np.random.default_rng(12)
+    inst, pol_o, pol_p = [], [], []
+    # exact ties: 2^18 terms of coefficient 1, half of them untouched by each RZ (so still 1 after it)
+    n = 40
+    m = 1 << 18
+    x = np.unique(rng.integers(0, 1 << 40, size=m + 4096, dtype=np.uint64))[:m].reshape(-1, 1)
+    z = np.zeros_like(x)
+    circ = Circuit(n).rz(0.3, 0).rz(0.4, 1).rz(0.5, 2).rz(0.6, 3)
+    inst.append((n, circ, (x, z, np.ones(len(x))), 1e-9))
+    pol_o.append(po.coeff_policy(eps=1e-9, topn=100_000))
+    pol_p.append(truncation.coeff(1e-9) & truncation.topn(100_000))
+    # the rest: brick-wall instances at their caps
+    for k, (nq, c, o, eps) in enumerate(circuits.batched_instances(seed=31, count=12, depths=(4, 9),
+                                                                      thresholds=(1e-5, 1e-3))):
+        cap = 150 + 40 * k
+        inst.append((nq, c, o, eps))
+        pol_o.append(po.coeff_policy(eps=eps, topn=cap))
+        pol_p.append(truncation.coeff(eps) & truncation.topn(cap))
+    check_batch(inst, pol_o, pol_p)
+    monkeypatch.setenv("PP_BATCH_TOPN", "0")
+    check_batch(inst, pol_o, pol_p)

@@ -363,7 +363,7 @@ struct pp_sum {
     uint64_t phys_len = 0;
     const uint8_t* pending_flags = nullptr;
     const float* pending_score = nullptr;  // the ranking score when the selection was by score
-    std::vector<uint64_t> tie_kept;  // TopN's kept boundary positions, the source of an asynchronous copy
+    std::vector<uint64_t> tie_kept;  // TopN's boundary positions in rank order, the source of an asynchronous copy
     uint64_t* d_tiles = nullptr;  // scratch of the bucket-offset scan
     uint64_t tiles_cap = 0;
     unsigned long long* d_stats = nullptr;
@@ -443,8 +443,11 @@ pp_status ensure_columns(pp_sum* s, int which, uint64_t need) {
     Columns& c = s->col[which];
     if (c.cap >= need && c.x) return PP_OK;
     free_columns(c);
-    // a multiple of kSegAlign, so that every word column starts as aligned as the first
-    const uint64_t cap = (std::max<uint64_t>(need + need / 4, 1024) + (kSegAlign - 1)) & ~(uint64_t)(kSegAlign - 1);
+    // Headroom of a quarter, and at small sizes of as much again (up to 2^20 rows): a growing sum then reallocates a
+    // few times rather than every few layers (each hipFree waits for the device).  A multiple of kSegAlign, so that
+    // every word column starts as aligned as the first.
+    const uint64_t extra = std::max<uint64_t>(need / 4, std::min<uint64_t>(need, 1ull << 20));
+    const uint64_t cap = (std::max<uint64_t>(need + extra, 1024) + (kSegAlign - 1)) & ~(uint64_t)(kSegAlign - 1);
     pp_status st;
     if ((st = dev_alloc(s->ctx, (void**)&c.x, cap * s->W * 8)) != PP_OK) return st;
     if ((st = dev_alloc(s->ctx, (void**)&c.z, cap * s->W * 8)) != PP_OK) return st;
@@ -457,7 +460,7 @@ pp_status ensure_buckets(pp_sum* s, int which, uint64_t need) {
     BucketArrays& b = s->bk[which];
     if (b.cap >= need && b.start) return PP_OK;
     free_buckets(b);
-    const uint64_t cap = std::max<uint64_t>(need, 256);
+    const uint64_t cap = std::max<uint64_t>(2 * need, 256);  // room for the next refinement's count
     pp_status st;
     if ((st = dev_alloc(s->ctx, (void**)&b.start, cap * 8)) != PP_OK) return st;
     if ((st = dev_alloc(s->ctx, (void**)&b.len, cap * 4)) != PP_OK) return st;
@@ -525,7 +528,7 @@ pp_status slot_scratch(pp_sum* s, ScratchSlot slot, size_t bytes, P** out) {
         if (s->scratch[slot]) (void)hipFree(s->scratch[slot]);
         s->scratch[slot] = nullptr;
         s->scratch_cap[slot] = 0;
-        const size_t grown = std::max<size_t>(256, bytes + bytes / 2);
+        const size_t grown = std::max<size_t>(256, 2 * bytes);
         pp_status st = dev_alloc(s->ctx, &s->scratch[slot], grown);
         if (st != PP_OK) return st;
         s->scratch_cap[slot] = grown;
@@ -1175,8 +1178,8 @@ pp_status launch_join(pp_sum* src, pp_sum* dst, int out, const Prepared& p, cons
         if (dst->d_redo) (void)hipFree(dst->d_redo);
         dst->d_redo = nullptr;
         dst->redo_cap = 0;
-        if ((st = dev_alloc(ctx, (void**)&dst->d_redo, (size_t)B * 4)) != PP_OK) return st;
-        dst->redo_cap = B;
+        if ((st = dev_alloc(ctx, (void**)&dst->d_redo, (size_t)B * 8)) != PP_OK) return st;
+        dst->redo_cap = 2ull * B;  // and the next refinement's
     }
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     const bool timed = ctx->profiling && !ctx->capturing && !fixed;
@@ -1295,8 +1298,8 @@ pp_status layer_in_place(pp_sum* s, const Prepared& p, const KeepProgram& keep) 
         if (s->d_redo) (void)hipFree(s->d_redo);
         s->d_redo = nullptr;
         s->redo_cap = 0;
-        if ((st = dev_alloc(ctx, (void**)&s->d_redo, (size_t)B * 4)) != PP_OK) return st;
-        s->redo_cap = B;
+        if ((st = dev_alloc(ctx, (void**)&s->d_redo, (size_t)B * 8)) != PP_OK) return st;
+        s->redo_cap = 2ull * B;  // and the next refinement's
     }
     PP_HIP(ctx, launch_fill_u64(ctx->stream, s->d_stats, kStatSlots, 0ull));
     const bool timed = ctx->profiling && !ctx->capturing;
@@ -1881,16 +1884,31 @@ struct RunPlanner {
 
     const pp_channel& channel_at(uint64_t i) const { return channels[adjoint ? n_channels - 1 - i : i]; }
 
+    // Layer i's prepared table.  The planner, the run and the layer loop ask for the same layers within a few steps
+    // of each other, so the last ones are kept (direct-mapped by layer index; a table depends on the sum's width,
+    // register and coefficient type only).
+    static constexpr uint32_t kPrepCache = 16;
+    mutable std::vector<Prepared> prep_cache = std::vector<Prepared>(kPrepCache);
+    mutable std::vector<uint64_t> prep_index = std::vector<uint64_t>(kPrepCache, UINT64_MAX);
+    Prepared prepared(pp_sum* s, uint64_t i) const {
+        const uint32_t slot = (uint32_t)(i % kPrepCache);
+        if (prep_index[slot] != i) {
+            prep_cache[slot] = prepare(s, channel_at(i), adjoint);
+            prep_index[slot] = i;
+        }
+        return prep_cache[slot];
+    }
+
     // The reference's count for the run starting at layer i, and the run's end.
     uint64_t start_run(pp_sum* s, uint64_t i) const {
         const int want = rebucketed_bits(s->len, s->reference_bits, target, min_buckets);
         s->reference_bits = std::min(want, s->max_bits);
-        const Prepared p = prepare(s, channel_at(i), adjoint);
+        const Prepared p = prepared(s, i);
         uint64_t end = i + 1;
         if (limit == 0) return end;
         if (s->len <= limit && is_permutation_step(p)) {
             uint64_t j = i + 1;
-            while (j < n_channels && is_permutation_step(prepare(s, channel_at(j), adjoint))) ++j;
+            while (j < n_channels && is_permutation_step(prepared(s, j))) ++j;
             if (j > i + 1) return j;
         }
         auto mul = [](uint64_t a, uint64_t b) { return (a != 0 && b > UINT64_MAX / a) ? UINT64_MAX : a * b; };
@@ -1899,7 +1917,7 @@ struct RunPlanner {
             std::vector<uint32_t> deltas{0, bucket_of_host(s, channel_at(i).gx, channel_at(i).gz, s->reference_bits)};
             uint64_t j = i + 1;
             while (j < n_channels && j < i + kRunLayers) {
-                const Prepared q = prepare(s, channel_at(j), adjoint);
+                const Prepared q = prepared(s, j);
                 const uint64_t grown = mul(worst, fanout(q));
                 if (grown > limit || !is_rotation_table(q)) break;
                 auto next = deltas;
@@ -1922,19 +1940,19 @@ struct RunPlanner {
         bool walk = end > i + 1 && s->len > 0 && fused_permutation_runs();
         std::vector<Prepared> steps;
         for (uint64_t j = i; walk && j < end; ++j) {
-            steps.push_back(prepare(s, channel_at(j), adjoint));
+            steps.push_back(prepared(s, j));
             walk = is_permutation_step(steps.back());
         }
         if (walk) return permutation_run_device(s, steps.data(), (uint32_t)steps.size(), keep, len_after);
         if (end - i >= kClassMinRunLayers && s->len > 0 && class_rotation_runs()) {
             std::vector<Prepared> run;
-            for (uint64_t j = i; j < end; ++j) run.push_back(prepare(s, channel_at(j), adjoint));
+            for (uint64_t j = i; j < end; ++j) run.push_back(prepared(s, j));
             bool done = false;
             if ((st = class_pass(s, run.data(), (uint32_t)run.size(), keep, len_after, &done)) != PP_OK) return st;
             if (done) return PP_OK;
         }
         for (uint64_t j = i; j < end; ++j) {
-            const Prepared q = prepare(s, channel_at(j), adjoint);
+            const Prepared q = prepared(s, j);
             if ((st = apply_one(s, q, keep)) != PP_OK) return st;
             if (len_after) len_after[j - i] = s->len;
         }
@@ -1955,7 +1973,7 @@ struct RunPlanner {
         std::vector<uint32_t> qs;
         uint64_t j = i;
         for (; j < n_channels && j < i + kClassMaxLayers; ++j) {
-            const Prepared q = prepare(s, channel_at(j), adjoint);
+            const Prepared q = prepared(s, j);
             if (!is_rotation_table(q) || is_permutation_step(q)) break;
             std::vector<uint32_t> more = qs;
             for (uint32_t k = 0; k < q.table.k; ++k) more.push_back(q.table.qubits[k]);
@@ -2894,10 +2912,10 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
     };
     while (i < n) {
         if (queued == 0 && i == run_end) run_end = plan.start_run(s, i);
-        const Prepared p = prepare(s, plan.channel_at(i), plan.adjoint);
+        const Prepared p = plan.prepared(s, i);
         bool lag = p.rotation && !p.key_preserving;
         if (lag && is_permutation_step(p) && i + 1 < n &&
-            is_permutation_step(prepare(s, plan.channel_at(i + 1), plan.adjoint)))
+            is_permutation_step(plan.prepared(s, i + 1)))
             lag = false;  // the start of a permutation run: walked in one pass
         if (lag && queued == 0) {
             // exact counts: the device bucketing for the layers to come, fine enough that a pair stays within the
@@ -3417,7 +3435,7 @@ pp_status pp_batch_propagate(pp_batch* b, const pp_channel* channels, const uint
             if (next[i] == run_end[i]) run_end[i] = plan.start_run(s, next[i]);
             const pp_channel& ch = plan.channel_at(next[i]);
             if (run_end[i] - next[i] >= 2 && s->len > 0 && fused_permutation_runs() &&
-                is_permutation_step(prepare(s, ch, plan.adjoint))) {
+                is_permutation_step(plan.prepared(s, next[i]))) {
                 // a permutation run: walked in one pass for this instance (RunPlanner::apply_run), not step by step
                 // in the batched join
                 if ((st = plan.apply_run(s, next[i], run_end[i], keeps[i], nullptr)) != PP_OK) return st;
@@ -3426,7 +3444,7 @@ pp_status pp_batch_propagate(pp_batch* b, const pp_channel* channels, const uint
                 continue;
             }
             if ((st = settle_device_bits(s)) != PP_OK) return st;
-            const Prepared p = prepare(s, ch, plan.adjoint);
+            const Prepared p = plan.prepared(s, next[i]);
             if (p.key_preserving) {  // in place, on its own
                 if ((st = device_layer(s, p, keeps[i])) != PP_OK) return st;
                 if (++next[i] == run_end[i] && (st = plan.finish_run(s, next[i] < plan.n_channels)) != PP_OK)

@@ -158,6 +158,9 @@ hipError_t launch_topn_gather_ties(hipStream_t st, const TopnTieArgs<W, T>& a, u
 template <int W, typename T>
 hipError_t launch_topn_retain(hipStream_t st, const TopnRetainArgs<W, T>& a, uint32_t num_buckets);
 hipError_t launch_mark_positions(hipStream_t st, const uint64_t* pos, uint64_t n, uint8_t* flags);
+// flags[pos[j]] = j < kept for j < ties: a boundary bin's positions in rank order.  Only positions in the boundary
+// bin are read (topn_keeps), so no other flag needs a value.
+hipError_t launch_topn_flag_ties(hipStream_t st, const uint64_t* pos, uint64_t ties, uint64_t kept, uint8_t* flags);
 
 // ---- ApproxTopN (truncation/builtin.rs:241-468): the cut at an octave of |c|^2 ----------------------------------
 

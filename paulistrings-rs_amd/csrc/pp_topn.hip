@@ -432,6 +432,12 @@ __global__ void k_mark_positions(const uint64_t* pos, uint64_t n, uint8_t* flags
     if (i < n) flags[pos[i]] = 1;
 }
 
+// The boundary flags of one selection: pos holds its boundary bin's positions in rank order, the first `kept` kept.
+__global__ void k_topn_flag_ties(const uint64_t* pos, uint64_t ties, uint64_t kept, uint8_t* flags) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < ties) flags[pos[j]] = j < kept ? 1 : 0;
+}
+
 // The boundary flags of many selections, one workgroup each: every position of the item's boundary bin, in rank
 // order, is flagged kept (its first `kept`) or dropped.  Only positions in the boundary bin are ever read, so no
 // other flag needs a value.
@@ -598,6 +604,13 @@ hipError_t launch_topn_gather_batch(hipStream_t st, const TopnTieArgs<W, T>* ite
     if (num_items == 0 || blocks == 0) return hipSuccess;
     hipLaunchKernelGGL((k_topn_gather_batch<W, T>), dim3(blocks), dim3(kBlock), 0, st, items, block_start,
                        num_items);
+    return hipGetLastError();
+}
+
+hipError_t launch_topn_flag_ties(hipStream_t st, const uint64_t* pos, uint64_t ties, uint64_t kept, uint8_t* flags) {
+    if (ties == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_topn_flag_ties, dim3((uint32_t)((ties + 255) / 256)), dim3(256), 0, st, pos, ties, kept,
+                       flags);
     return hipGetLastError();
 }
 

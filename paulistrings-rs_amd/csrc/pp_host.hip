@@ -425,17 +425,15 @@ pp_status dev_alloc(pp_context* ctx, void** p, size_t bytes) {
     return PP_OK;
 }
 
+// A column set is one allocation (x, then z, then the coefficients), and so are a layout's bucket arrays (start,
+// then len, then rows): one allocation call, and one free, each.
 void free_columns(Columns& c) {
     if (c.x) (void)hipFree(c.x);
-    if (c.z) (void)hipFree(c.z);
-    if (c.c) (void)hipFree(c.c);
     c = Columns{};
 }
 
 void free_buckets(BucketArrays& b) {
     if (b.start) (void)hipFree(b.start);
-    if (b.len) (void)hipFree(b.len);
-    if (b.rows) (void)hipFree(b.rows);
     b = BucketArrays{};
 }
 
@@ -449,9 +447,11 @@ pp_status ensure_columns(pp_sum* s, int which, uint64_t need) {
     const uint64_t extra = std::max<uint64_t>(need / 4, std::min<uint64_t>(need, 1ull << 20));
     const uint64_t cap = (std::max<uint64_t>(need + extra, 1024) + (kSegAlign - 1)) & ~(uint64_t)(kSegAlign - 1);
     pp_status st;
-    if ((st = dev_alloc(s->ctx, (void**)&c.x, cap * s->W * 8)) != PP_OK) return st;
-    if ((st = dev_alloc(s->ctx, (void**)&c.z, cap * s->W * 8)) != PP_OK) return st;
-    if ((st = dev_alloc(s->ctx, &c.c, cap * s->coeff_bytes())) != PP_OK) return st;
+    unsigned char* base = nullptr;
+    if ((st = dev_alloc(s->ctx, (void**)&base, cap * (2 * s->W * 8 + s->coeff_bytes()))) != PP_OK) return st;
+    c.x = (uint64_t*)base;
+    c.z = (uint64_t*)(base + cap * s->W * 8);  // cap is a multiple of kSegAlign: every array starts 128-B aligned
+    c.c = base + 2 * cap * s->W * 8;
     c.cap = cap;
     return PP_OK;
 }
@@ -462,9 +462,11 @@ pp_status ensure_buckets(pp_sum* s, int which, uint64_t need) {
     free_buckets(b);
     const uint64_t cap = std::max<uint64_t>(2 * need, 256);  // room for the next refinement's count
     pp_status st;
-    if ((st = dev_alloc(s->ctx, (void**)&b.start, cap * 8)) != PP_OK) return st;
-    if ((st = dev_alloc(s->ctx, (void**)&b.len, cap * 4)) != PP_OK) return st;
-    if ((st = dev_alloc(s->ctx, (void**)&b.rows, cap * 4)) != PP_OK) return st;
+    unsigned char* base = nullptr;
+    if ((st = dev_alloc(s->ctx, (void**)&base, cap * 16)) != PP_OK) return st;
+    b.start = (uint64_t*)base;
+    b.len = (uint32_t*)(base + cap * 8);
+    b.rows = (uint32_t*)(base + cap * 12);
     b.cap = cap;
     return PP_OK;
 }

@@ -373,6 +373,7 @@ struct pp_sum {
     unsigned long long* h_lag = nullptr;
     unsigned long long* d_lag_host = nullptr;
     unsigned long long lag_seq = 0;
+    unsigned long long* d_lag_counters = nullptr;  // the small joins' counters: two sets of 4, by ring slot
     pp::TopnState* d_topn = nullptr;
     unsigned long long* d_hist = nullptr;
     uint32_t* d_redo = nullptr;  // pairs a join could not stage, for its redo launch
@@ -2269,6 +2270,7 @@ pp_status pp_sum_free(pp_sum* s) {
     if (s->d_stats) (void)hipFree(s->d_stats);
     if (s->h_stats) (void)hipHostFree(s->h_stats);
     if (s->h_lag) (void)hipHostFree(s->h_lag);
+    if (s->d_lag_counters) (void)hipFree(s->d_lag_counters);
     if (s->d_topn) (void)hipFree(s->d_topn);
     if (s->d_redo) (void)hipFree(s->d_redo);
     if (s->d_score) (void)hipFree(s->d_score);
@@ -2822,10 +2824,12 @@ bool small_join_fits(const pp_sum* s, const Prepared& p) {
     return join_stage_rows(s->ctx->tuning.stage_rows, lcap) >= lcap;
 }
 
-// Queue the small join of layer p from s's current columns into column set `out`: the counters arrive in the
-// pinned slot whose device address is `host`, followed by `seq`.  s's counters (stats 0, 1, 2 and 5) are zero.
+// Queue the small join of layer p from s's current columns into column set `out`, adding into `counters` (zero);
+// its extra workgroup publishes `prev` (the counters of the small join queued just before, or null) to the pinned
+// slot whose device address is `prev_host`, followed by `prev_seq`.
 pp_status launch_join_small(pp_sum* s, int out, const Prepared& p, const KeepProgram& keep,
-                            unsigned long long* host, unsigned long long seq) {
+                            unsigned long long* counters, unsigned long long* prev, unsigned long long* prev_host,
+                            unsigned long long prev_seq) {
     pp_context* ctx = s->ctx;
     const uint32_t B = 1u << s->device_bits;
     const uint32_t delta = bucket_of_host(s, p.gx, p.gz, s->device_bits);
@@ -2835,12 +2839,13 @@ pp_status launch_join_small(pp_sum* s, int out, const Prepared& p, const KeepPro
         using T = decltype(Tv);
         SmallJoinArgs<W, T> a{};
         a.j = join_args<W, T>(s, s, out, p, keep, delta, lcap);
-        a.j.stats = s->d_stats;
+        a.j.stats = counters;
         a.j.count_totals = 1;
         a.orows = s->bk[out].rows;
         a.num_buckets = B;
-        a.host = host;
-        a.seq = seq;
+        a.prev_counters = prev;
+        a.prev_host = prev_host;
+        a.prev_seq = prev_seq;
         return launch_rotation_join_small<W, T>(ctx->stream, a);
     });
     PP_HIP(ctx, e);
@@ -2861,13 +2866,27 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
         std::memset(s->h_lag, 0, 2 * kStatSlots * sizeof(unsigned long long));
         PP_HIP(ctx, hipHostGetDevicePointer((void**)&s->d_lag_host, s->h_lag, 0));
     }
+    if (!s->d_lag_counters) {  // zero from here on: each set is cleared when its counters are published
+        if ((st = dev_alloc(ctx, (void**)&s->d_lag_counters, 8 * sizeof(unsigned long long))) != PP_OK) return st;
+        PP_HIP(ctx, launch_fill_u64(ctx->stream, s->d_lag_counters, 8, 0ull));
+    }
     const uint64_t n = plan.n_channels;
     const uint64_t target = ctx->tuning.bucket_target;
     uint64_t i = 0, run_end = 0;  // layer i is the next to queue; the planner's current run ends at run_end
     uint32_t queued = 0, head = 0;  // layers whose counters are on their way; ring slot of the oldest
     uint64_t queued_layer[2] = {};
     unsigned long long queued_seq[2] = {};  // a small join's sequence number in its slot, or 0 (a copied slot)
-    bool counters_clear = false;  // s's counters are zero (a small join leaves them so)
+    int unpublished = -1;  // the slot of a small join whose counters no launch publishes yet
+    auto counters_of = [&](int k) { return s->d_lag_counters + 4 * k; };
+    auto host_of = [&](int k) { return s->d_lag_host + kStatSlots * k; };
+    // the last small join's counters, when no small join follows it to publish them
+    auto publish = [&]() -> pp_status {
+        if (unpublished < 0) return PP_OK;
+        PP_HIP(ctx, launch_publish_counters(ctx->stream, counters_of(unpublished), host_of(unpublished),
+                                            queued_seq[unpublished]));
+        unpublished = -1;
+        return PP_OK;
+    };
     uint64_t known_len = s->len, known_mb = s->max_bucket;  // counts after the last layer read
     // the planner's bookkeeping at layer r, once the length before it is known
     auto plan_at = [&](uint64_t r, uint64_t len_before) {
@@ -2881,12 +2900,13 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
         const uint32_t k = head;
         volatile const unsigned long long* h = s->h_lag + kStatSlots * k;
         if (queued_seq[k]) {
-            // the small join's last workgroup writes the slot, then the sequence number: wait for it on the host,
-            // and after a long wait for the launch itself (which reports a failed launch)
+            // the next launch (a small join, or the publication at a drain) writes the slot, then the sequence
+            // number: wait for it on the host, and after a long wait for the stream itself (which reports a failed
+            // launch)
             const auto t0 = std::chrono::steady_clock::now();
             while (h[4] != queued_seq[k])
                 if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
-                    PP_HIP(ctx, hipEventSynchronize(ctx->lag_ev[k]));
+                    PP_HIP(ctx, hipStreamSynchronize(ctx->stream));
                     if (h[4] != queued_seq[k]) return fail(ctx, PP_ERR_DRIVER, "small join: counters never arrived");
                     break;
                 }
@@ -2904,6 +2924,8 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
         return PP_OK;
     };
     auto drain = [&]() -> pp_status {
+        pp_status ps = publish();
+        if (ps != PP_OK) return ps;
         while (queued) {
             pp_status rs = read_oldest();
             if (rs != PP_OK) return rs;
@@ -2928,7 +2950,6 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
                 if ((st = refine_once(s)) != PP_OK) return st;
             known_len = s->len;
             known_mb = s->max_bucket;
-            counters_clear = false;  // a rebucketing counts with them
         }
         const uint64_t len_hi = known_len << queued, mb_hi = known_mb << queued;
         lag = lag && len_hi <= kLagMaxRows && 2 * mb_hi <= kMaxPairRows &&
@@ -2957,7 +2978,6 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
             if ((st = debug_check(s)) != PP_OK) return st;
             known_len = s->len;
             known_mb = s->max_bucket;
-            counters_clear = false;
             i = end;
             continue;
         }
@@ -2965,18 +2985,21 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
         s->len = len_hi;
         s->max_bucket = (uint32_t)mb_hi;
         const uint32_t k = (head + queued) & 1u;
-        if (small_join_fits(s, p)) {  // one launch: the join lays out its output and publishes its counters
-            if (!counters_clear) PP_HIP(ctx, launch_fill_u64(ctx->stream, s->d_stats, kStatSlots, 0ull));
+        if (small_join_fits(s, p)) {  // one launch: the join lays out its output, and publishes the one before
+            const int prev = unpublished == (int)(k ^ 1u) ? (int)(k ^ 1u) : -1;
+            if (unpublished >= 0 && prev < 0 && (st = publish()) != PP_OK) return st;
             queued_seq[k] = ++s->lag_seq;
-            if ((st = launch_join_small(s, out, p, keep, s->d_lag_host + kStatSlots * k, queued_seq[k])) != PP_OK)
+            if ((st = launch_join_small(s, out, p, keep, counters_of((int)k), prev >= 0 ? counters_of(prev) : nullptr,
+                                        prev >= 0 ? host_of(prev) : nullptr,
+                                        prev >= 0 ? queued_seq[prev] : 0ull)) != PP_OK)
                 return st;
-            counters_clear = true;
+            unpublished = (int)k;
         } else {
+            if ((st = publish()) != PP_OK) return st;
             if ((st = launch_join(s, s, out, p, keep)) != PP_OK) return st;
             PP_HIP(ctx, hipMemcpyAsync(s->h_lag + kStatSlots * k, s->d_stats, kStatSlots * sizeof(unsigned long long),
                                        hipMemcpyDeviceToHost, ctx->stream));
             queued_seq[k] = 0;
-            counters_clear = false;
         }
         PP_HIP(ctx, hipEventRecord(ctx->lag_ev[k], ctx->stream));
         queued_layer[k] = i;

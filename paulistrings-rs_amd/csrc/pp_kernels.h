@@ -52,18 +52,21 @@ struct JoinArgs {
 };
 
 // A small layer's join in one launch (k_rotation_join_small, a layout of at most kSmallJoinBuckets buckets, one
-// workgroup per pair): every workgroup lays out its own pair's output segments from the input lengths, joins the
-// pair, and the last workgroup to finish publishes the layer's counters to a pinned host slot and clears them, and
-// its finish counter stats[5], for the next launch.  j.stats[0, 1, 2, 5] are zero before the launch.
+// workgroup per pair): every workgroup lays out its own pair's output segments from the input lengths and joins the
+// pair, adding its rows and longest bucket to j.stats[0, 1] (zero before the launch; [2] flags an overflow, and
+// workgroup 0 writes the layout's end to [3]).  One workgroup more publishes the counters of the launch before
+// (prev_counters, complete at this launch's start) to that launch's pinned host slot, then a sequence number the
+// host waits on, and clears them for the launch after this one.
 constexpr uint32_t kSmallJoinBuckets = 2048;
 template <int W, typename T>
 struct SmallJoinArgs {
     JoinArgs<W, T> j;          // j.ostart and j.olen are written; j.count_totals set; no redo queue
     uint32_t* orows;           // each output segment's room in rows
     uint32_t num_buckets;
-    unsigned long long* host;  // pinned, device-visible: [0] rows, [1] longest bucket, [2] overflow, [3] layout end,
-                               // [4] seq once [0 .. 3] are written
-    unsigned long long seq;
+    unsigned long long* prev_counters;  // 4 counters of the launch before, or null
+    unsigned long long* prev_host;      // pinned, device-visible: [0] rows, [1] longest bucket, [2] overflow, [3]
+                                        // layout end, [4] prev_seq once [0 .. 3] are written
+    unsigned long long prev_seq;
 };
 
 // The rotation layer in place (k_rotation_inplace): the sum's own columns and bucket arrays are read and written.
@@ -294,6 +297,10 @@ template <int W, typename T>
 hipError_t launch_rotation_join(hipStream_t st, const JoinArgs<W, T>& a, uint32_t groups_per_cu, uint32_t block);
 template <int W, typename T>
 hipError_t launch_rotation_join_small(hipStream_t st, const SmallJoinArgs<W, T>& a);
+// The publication of a small join's counters with no launch after it (one workgroup): as the extra workgroup of
+// k_rotation_join_small does.
+hipError_t launch_publish_counters(hipStream_t st, unsigned long long* counters, unsigned long long* host,
+                                   unsigned long long seq);
 // One layer for each of num_items sums (device arrays: items, pair_start = exclusive prefix of their pair counts,
 // total_pairs in all): their output offsets, the joins (each item staging its largest pair, scap = lcap), and
 // their counters, also gathered into out[4 i + 0..2] = rows, longest bucket, overflow flag.

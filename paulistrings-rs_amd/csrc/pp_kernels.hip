@@ -638,41 +638,49 @@ __global__ void __launch_bounds__(BLOCK) k_rotation_join(const JoinArgs<W, T> a)
     join_items<W, T, REDO, BLOCK, FILTER>(a, blockIdx.x, gridDim.x, count, lds, scan_scratch);
 }
 
+// A small layer's counters to the host: the four words, then (once they have landed) the sequence number the host
+// waits on; then the counters are cleared for the next launch that adds into them.  Thread 0 of one workgroup.
+__device__ __forceinline__ void publish_counters(unsigned long long* counters, unsigned long long* host,
+                                                 unsigned long long seq) {
+    volatile unsigned long long* c = counters;
+    volatile unsigned long long* h = host;  // fine-grained host memory: the stores bypass the L2
+    const unsigned long long rows = c[0], longest = c[1], flags = c[2], end = c[3];
+    h[0] = rows;
+    h[1] = longest;
+    h[2] = flags;
+    h[3] = end;
+    c[0] = 0;
+    c[1] = 0;
+    c[2] = 0;
+    c[3] = 0;
+    __builtin_amdgcn_s_waitcnt(0);  // the slot's words land before its sequence number
+    h[4] = seq;
+}
+
 // A small layer's join in one launch (see SmallJoinArgs).  The workgroup of pair p sums the output segment sizes
 // (each pair's rows rounded up to kSegAlign, as the offset scan lays them out) below each of its two buckets and in
 // all, over every bucket of the layout (at most kSmallJoinBuckets, read by every workgroup while its pair's first
-// rows load), writes its buckets' starts and rooms, and joins the pair.  The last workgroup to finish (a finish counter in stats[5])
-// copies the counters to the host slot, then its sequence number, and clears them for the next launch.
+// rows load), writes its buckets' starts and rooms, and joins the pair.  The workgroup after the last pair
+// publishes the launch before's counters, which that launch's end made complete: a layer's counters reach the host
+// while the next layer runs, and no workgroup waits on another's.
 template <int W, typename T>
 __global__ void __launch_bounds__(kBlock) k_rotation_join_small(const SmallJoinArgs<W, T> a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t scan_scratch[4 * kWaves];
     __shared__ uint64_t part[3][kWaves];
     const JoinArgs<W, T>& j = a.j;
-    uint64_t all = 0;
-    join_items<W, T, false, kBlock, false, true>(j, blockIdx.x, gridDim.x, j.num_cosets, lds, scan_scratch,
-                                                 a.num_buckets, a.orows, &part[0][0], &all);
-    // Only atomics order the finish: the counters are device-scope atomics, done (waited for) before this
-    // workgroup's finish is counted, and the last workgroup reads them back with atomics.  No fence: a device-scope
-    // fence in every workgroup writes back its L2.  The rows and lengths reach the next kernel at the kernel
-    // boundary; the host reads only the slot.
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_s_waitcnt(0);
-        if (atomicAdd(&j.stats[5], 1ull) == gridDim.x - 1ull) {
-            const unsigned long long rows = atomicExch(&j.stats[0], 0ull);
-            const unsigned long long longest = atomicExch(&j.stats[1], 0ull);
-            const unsigned long long flags = atomicExch(&j.stats[2], 0ull);
-            atomicExch(&j.stats[5], 0ull);
-            j.stats[3] = all;
-            volatile unsigned long long* h = a.host;  // fine-grained host memory: the stores bypass the L2
-            h[0] = rows;
-            h[1] = longest;
-            h[2] = flags;
-            h[3] = all;
-            __builtin_amdgcn_s_waitcnt(0);  // the slot's words land before its sequence number
-            h[4] = a.seq;
-        }
+    if (blockIdx.x == j.num_cosets) {  // uniform: the publishing workgroup
+        if (threadIdx.x == 0 && a.prev_counters) publish_counters(a.prev_counters, a.prev_host, a.prev_seq);
+        return;
     }
+    uint64_t all = 0;
+    join_items<W, T, false, kBlock, false, true>(j, blockIdx.x, j.num_cosets, j.num_cosets, lds, scan_scratch,
+                                                 a.num_buckets, a.orows, &part[0][0], &all);
+    if (blockIdx.x == 0 && threadIdx.x == 0) j.stats[3] = all;
+}
+
+__global__ void k_publish_counters(unsigned long long* counters, unsigned long long* host, unsigned long long seq) {
+    if (threadIdx.x == 0) publish_counters(counters, host, seq);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1575,7 +1583,13 @@ hipError_t launch_rotation_join_small(hipStream_t st, const SmallJoinArgs<W, T>&
     if (a.j.num_cosets == 0 || a.num_buckets > kSmallJoinBuckets || a.j.redo_mode || a.j.filter || !a.j.count_totals)
         return hipErrorInvalidValue;
     const size_t lds = join_lds_bytes<W, T>(a.j.scap, a.j.tcap_bits);
-    hipLaunchKernelGGL((k_rotation_join_small<W, T>), dim3(a.j.num_cosets), dim3(kBlock), lds, st, a);
+    hipLaunchKernelGGL((k_rotation_join_small<W, T>), dim3(a.j.num_cosets + 1), dim3(kBlock), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_publish_counters(hipStream_t st, unsigned long long* counters, unsigned long long* host,
+                                   unsigned long long seq) {
+    hipLaunchKernelGGL(k_publish_counters, dim3(1), dim3(64), 0, st, counters, host, seq);
     return hipGetLastError();
 }
 

@@ -2999,9 +2999,9 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
             if ((st = launch_join(s, s, out, p, keep)) != PP_OK) return st;
             PP_HIP(ctx, hipMemcpyAsync(s->h_lag + kStatSlots * k, s->d_stats, kStatSlots * sizeof(unsigned long long),
                                        hipMemcpyDeviceToHost, ctx->stream));
+            PP_HIP(ctx, hipEventRecord(ctx->lag_ev[k], ctx->stream));  // a small join's slot carries its own mark
             queued_seq[k] = 0;
         }
-        PP_HIP(ctx, hipEventRecord(ctx->lag_ev[k], ctx->stream));
         queued_layer[k] = i;
         ++queued;
         s->cur = out;

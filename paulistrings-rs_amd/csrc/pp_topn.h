@@ -145,7 +145,9 @@ hipError_t launch_topn_flags_batch(hipStream_t st, const TopnFlagItem* items, ui
 template <typename T>
 hipError_t launch_topn_hist(hipStream_t st, const TopnHistArgs<T>& a, uint32_t blocks);
 hipError_t launch_topn_select(hipStream_t st, TopnState* state, unsigned long long* hist, int pass, uint64_t n);
-hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n, uint64_t seed = 0, bool seeded = false);
+// A new selection's state, its histogram (hist_words words) cleared.
+hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n, unsigned long long* hist,
+                            uint32_t hist_words, uint64_t seed = 0, bool seeded = false);
 // The seeded first pass: besides the first digit's histogram, the second digit's over the terms whose first digit is
 // the seed's; the select then decides one or two digits (the next pass does nothing for a digit decided).  hist holds
 // kSeededHistWords.

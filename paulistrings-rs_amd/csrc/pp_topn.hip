@@ -14,7 +14,12 @@ __device__ __forceinline__ uint64_t rank_bits(const T* c, const float* score, ui
     return score ? score_bits(score[i]) : magnitude_bits<T>(c[i]);
 }
 
-__global__ void k_topn_init(TopnState* s, uint64_t n, uint64_t seed, uint32_t seeded) {
+// The state of a new selection, and (one workgroup) its histogram cleared: ApproxTopN's octave pass, which shares
+// it, leaves its counts behind.
+__global__ void __launch_bounds__(kBlock) k_topn_init(TopnState* s, uint64_t n, uint64_t seed, uint32_t seeded,
+                                                      unsigned long long* hist, uint32_t hist_words) {
+    for (uint32_t i = threadIdx.x; i < hist_words; i += kBlock) hist[i] = 0ull;
+    if (threadIdx.x != 0) return;
     s->prefix = 0;
     s->mask = 0;
     s->k = n;
@@ -631,8 +636,10 @@ hipError_t launch_topn_select(hipStream_t st, TopnState* state, unsigned long lo
     return hipGetLastError();
 }
 
-hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n, uint64_t seed, bool seeded) {
-    hipLaunchKernelGGL(k_topn_init, dim3(1), dim3(1), 0, st, state, n, seed, seeded ? 1u : 0u);
+hipError_t launch_topn_init(hipStream_t st, TopnState* state, uint64_t n, unsigned long long* hist,
+                            uint32_t hist_words, uint64_t seed, bool seeded) {
+    hipLaunchKernelGGL(k_topn_init, dim3(1), dim3(kBlock), 0, st, state, n, seed, seeded ? 1u : 0u, hist,
+                       hist_words);
     return hipGetLastError();
 }
 

@@ -2866,10 +2866,12 @@ pp_status propagate_lagged(pp_sum* s, const RunPlanner& plan, const KeepProgram&
         std::memset(s->h_lag, 0, 2 * kStatSlots * sizeof(unsigned long long));
         PP_HIP(ctx, hipHostGetDevicePointer((void**)&s->d_lag_host, s->h_lag, 0));
     }
-    if (!s->d_lag_counters) {  // zero from here on: each set is cleared when its counters are published
-        if ((st = dev_alloc(ctx, (void**)&s->d_lag_counters, 8 * sizeof(unsigned long long))) != PP_OK) return st;
-        PP_HIP(ctx, launch_fill_u64(ctx->stream, s->d_lag_counters, 8, 0ull));
-    }
+    if (!s->d_lag_counters &&
+        (st = dev_alloc(ctx, (void**)&s->d_lag_counters, 8 * sizeof(unsigned long long))) != PP_OK)
+        return st;
+    // zero from here on, each set cleared when its counters are published (and here, in case a propagation that
+    // failed left one unpublished)
+    PP_HIP(ctx, launch_fill_u64(ctx->stream, s->d_lag_counters, 8, 0ull));
     const uint64_t n = plan.n_channels;
     const uint64_t target = ctx->tuning.bucket_target;
     uint64_t i = 0, run_end = 0;  // layer i is the next to queue; the planner's current run ends at run_end

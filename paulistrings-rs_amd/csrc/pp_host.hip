@@ -360,6 +360,7 @@ struct pp_sum {
     uint64_t class_out_hint = 0;   // the last class pass's outputs: the next one's first guess at its staging
     uint64_t topn_seed = 0;        // the last selection's boundary prefix, which seeds the next one's first pass
     bool topn_seed_valid = false;
+    bool topn_seed_score = false;  // the seed is a score key (the learned pass), not a |c|^2 one
     uint64_t phys_len = 0;
     const uint8_t* pending_flags = nullptr;
     const float* pending_score = nullptr;  // the ranking score when the selection was by score

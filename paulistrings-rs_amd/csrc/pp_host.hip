@@ -2798,8 +2798,10 @@ namespace {
 // ---- lagged lengths (engine/gpu/length.rs:1-4) ------------------------------------------------------------------
 //
 // A small rotation layer's exact length and longest bucket reach the host one layer late.  Each queued layer's
-// counters are copied to a pinned slot behind its kernels; the host waits for the previous layer's copy while the
-// current layer runs, so the GPU is not left idle across a host round trip per layer.  The next layer is sized from
+// counters reach a pinned slot of their own: a small join's from the next launch's extra workgroup (or, before a
+// drain, a one-workgroup publication), a larger layer's by a copy behind its kernels.  The host waits for the
+// previous layer's slot while the current layer runs, so the GPU is not left idle across a host round trip per
+// layer.  The next layer is sized from
 // bounds: a rotation layer at most doubles the rows, and an output bucket draws on two input buckets, so at most
 // doubles the longest bucket.  A layer whose bounds do not fit (a pair past the largest a join stages, a column set
 // that must grow), a sum large enough for the in-place layer, or a table that is not a rotation drains the queue and
